@@ -1,0 +1,147 @@
+// Device helpers shared by the walker kernels: the per-walker counter-based
+// random stream and precision-overloaded math.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ffr {
+
+// ---------------------------------------------------------------------------
+// Walker stream.  Word n of walker w under seed s is half (n & 1) of the
+// Philox4x32-10 block (Salmon et al., SC'11) with key (lo32 s, hi32 s) and
+// counter (lo32 (n>>1), hi32 (n>>1), lo32 w, hi32 w); half 0 = out0|out1<<32,
+// half 1 = out2|out3<<32.  This stream replaces the reference's clock-seeded
+// thread-local ISAAC (src/rng/flame_rng.hpp:29-36): every walker (reference
+// "batch", buffer_renderer.hpp:155) owns one, so renders are reproducible.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                              uint32_t c3, uint32_t k0, uint32_t k1,
+                                              uint32_t &o0, uint32_t &o1, uint32_t &o2,
+                                              uint32_t &o3)
+{
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    o0 = c0;
+    o1 = c1;
+    o2 = c2;
+    o3 = c3;
+}
+
+struct Stream {
+    uint32_t k0, k1, w0, w1;
+    uint64_t n;       // index of the next word
+    uint64_t cache;   // word cache_n, when loaded with its even partner
+    uint64_t cache_n;
+
+    __device__ __forceinline__ void init(uint64_t seed, uint64_t walker)
+    {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        w0 = (uint32_t)walker;
+        w1 = (uint32_t)(walker >> 32);
+        n = 0;
+        cache = 0;
+        cache_n = ~0ull;
+    }
+
+    __device__ __forceinline__ uint64_t next()
+    {
+        uint64_t w;
+        if (n == cache_n) {
+            w = cache;
+        } else {
+            uint64_t blk = n >> 1;
+            uint32_t o0, o1, o2, o3;
+            philox4x32_10((uint32_t)blk, (uint32_t)(blk >> 32), w0, w1, k0, k1, o0, o1,
+                          o2, o3);
+            uint64_t lo = (uint64_t)o0 | ((uint64_t)o1 << 32);
+            uint64_t hi = (uint64_t)o2 | ((uint64_t)o3 << 32);
+            if (n & 1) {
+                w = hi;
+            } else {
+                w = lo;
+                cache = hi;
+                cache_n = n + 1;
+            }
+        }
+        ++n;
+        return w;
+    }
+};
+
+// randNum (flame_rng.hpp:67-87): top 53 bits over 2^53 in double, top 24
+// bits over 2^24 in float; both exact
+__device__ __forceinline__ double rand_num(Stream &s, double)
+{
+    return (double)(s.next() >> 11) * 0x1p-53;
+}
+__device__ __forceinline__ float rand_num(Stream &s, float)
+{
+    return (float)(s.next() >> 40) * 0x1p-24f;
+}
+
+// ---------------------------------------------------------------------------
+// math overloads: the argument type picks the precision (C++ <cmath> rules)
+// ---------------------------------------------------------------------------
+#define FFR_M1(name, dfn, ffn)                                                   \
+    __device__ __forceinline__ double m_##name(double x) { return dfn(x); }      \
+    __device__ __forceinline__ float m_##name(float x) { return ffn(x); }
+FFR_M1(sin, sin, sinf)
+FFR_M1(cos, cos, cosf)
+FFR_M1(tan, tan, tanf)
+FFR_M1(sqrt, sqrt, sqrtf)
+FFR_M1(exp, exp, expf)
+FFR_M1(log, log, logf)
+FFR_M1(log10, log10, log10f)
+FFR_M1(sinh, sinh, sinhf)
+FFR_M1(cosh, cosh, coshf)
+FFR_M1(acos, acos, acosf)
+FFR_M1(floor, floor, floorf)
+FFR_M1(trunc, trunc, truncf)
+FFR_M1(rint, rint, rintf)
+FFR_M1(fabs, fabs, fabsf)
+#undef FFR_M1
+
+__device__ __forceinline__ double m_atan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float m_atan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ double m_pow(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float m_pow(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double m_fmod(double a, double b) { return fmod(a, b); }
+__device__ __forceinline__ float m_fmod(float a, float b) { return fmodf(a, b); }
+__device__ __forceinline__ double m_hypot(double a, double b) { return hypot(a, b); }
+__device__ __forceinline__ float m_hypot(float a, float b) { return hypotf(a, b); }
+__device__ __forceinline__ double m_copysign(double a, double b) { return copysign(a, b); }
+__device__ __forceinline__ void m_sincos(double a, double *s, double *c) { sincos(a, s, c); }
+__device__ __forceinline__ void m_sincos(float a, float *s, float *c) { sincosf(a, s, c); }
+
+template <typename R> struct Consts;
+template <> struct Consts<double> {
+    static constexpr double eps = 1e-20;           // constants.hpp:17-22
+    static constexpr double bad = 1e20;            // constants.hpp:50-56
+};
+template <> struct Consts<float> {
+    static constexpr float eps = 1e-10f;
+    static constexpr float bad = 1e10f;
+};
+
+template <typename R> __device__ __forceinline__ bool bad_value(R v)
+{
+    // utils/flame.hpp:26-29
+    return m_fabs(v) > Consts<R>::bad || v != v;
+}
+
+} // namespace ffr

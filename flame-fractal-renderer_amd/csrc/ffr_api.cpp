@@ -1,0 +1,642 @@
+// C ABI of the renderer (include/ffr.h): flame parsing, device buffers,
+// kernel launches and statistics.
+#include "../../include/ffr.h"
+
+#include "flame_model.hpp"
+#include "util_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace ffr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(FFR_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevTables {
+    void *table = nullptr;
+    unsigned long long *masks = nullptr;
+    size_t table_bytes = 0;
+    int tab_reals = 0;
+};
+
+} // namespace
+
+struct ffr_renderer {
+    ffr_options opt{};
+    FlameSpec spec;
+    int mode = MODE_F64_U64;
+    int device = 0;
+    int cu_count = 256;
+    DeviceFlame<double> f64;
+    DeviceFlame<float> f32;
+    DevTables tab;
+    int vs = 2;
+    size_t lds_bytes = 0;
+    // buffer
+    void *dbuf = nullptr;
+    uint64_t cells = 0;
+    uint32_t cell_words = 1;
+    uint32_t word_bytes = 8;
+    // statistics
+    DevStats *dstats = nullptr;
+    unsigned long long *dxf = nullptr;
+    unsigned long long *dbv_xfs = nullptr;
+    double *dbv_pts = nullptr;
+    unsigned long long *dhist = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t next_walker = 0;
+    // cumulative statistics (getters of the reference)
+    uint64_t s_iter = 0, s_plot = 0;
+    std::vector<uint64_t> xf_dist; // by xform id
+    double pmin[3], pmax[3];
+    // last render
+    uint64_t last_bad = 0, last_walkers = 0, last_batch = 0;
+    double last_ms = 0;
+    std::vector<uint64_t> bv_ids;
+    std::vector<double> bv_pts;
+    bool pending = false;
+    int nxf = 0;
+    std::vector<int> order; // sorted position -> xform id
+};
+
+static int pick_variation_set(const uint32_t ts[4])
+{
+    bool linear_only = true, basic = true;
+    for (int t = 0; t < kNumVariations; ++t) {
+        if (!(ts[t / 32] >> (t % 32) & 1u))
+            continue;
+        if (t != 0)
+            linear_only = false;
+        if (!(t == 0 || t == 1 || t == 2 || t == 20 || t == 21 || t == 22))
+            basic = false;
+    }
+    return linear_only ? 0 : (basic ? 1 : 2);
+}
+
+template <typename R>
+static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
+{
+    if ((int)d.slot_type.size() > kMaxSlots)
+        return fail(FFR_E_UNSUPPORTED, "too many variation slots");
+    if ((int)d.fin_type.size() > kMaxFinalVars)
+        return fail(FFR_E_UNSUPPORTED, "final xform has more than " +
+                                           std::to_string(kMaxFinalVars) + " variations");
+    if (d.nxf > kMaxXforms)
+        return fail(FFR_E_UNSUPPORTED, "more than " + std::to_string(kMaxXforms) + " xforms");
+    r->tab.tab_reals = (int)d.table.size();
+    r->tab.table_bytes = d.table.size() * sizeof(R);
+    HIP_TRY(hipMalloc(&r->tab.table, std::max<size_t>(r->tab.table_bytes, 16)));
+    HIP_TRY(hipMemcpy(r->tab.table, d.table.data(), r->tab.table_bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&r->tab.masks, sizeof(unsigned long long) * (size_t)d.nxf));
+    HIP_TRY(hipMemcpy(r->tab.masks, d.masks.data(), sizeof(unsigned long long) * (size_t)d.nxf,
+                      hipMemcpyHostToDevice));
+    r->lds_bytes = r->tab.table_bytes + sizeof(unsigned long long) * (size_t)d.nxf +
+                   sizeof(unsigned int) * (size_t)d.nxf;
+    int max_lds = 0;
+    HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                  r->device));
+    if (r->lds_bytes > (size_t)max_lds)
+        return fail(FFR_E_UNSUPPORTED, "flame table of " + std::to_string(r->lds_bytes) +
+                                           " bytes exceeds the " + std::to_string(max_lds) +
+                                           "-byte LDS of a workgroup");
+    r->vs = pick_variation_set(d.type_set);
+    r->nxf = d.nxf;
+    r->order = d.order;
+    return FFR_OK;
+}
+
+extern "C" {
+
+void ffr_default_options(ffr_options *opt)
+{
+    if (!opt)
+        return;
+    std::memset(opt, 0, sizeof *opt);
+    opt->precision = FFR_FP64;
+    opt->word = FFR_WORD64;
+    opt->bad_value_limit = 256;
+    opt->bad_value_cap = 4096;
+}
+
+const char *ffr_last_error(void) { return g_err.c_str(); }
+
+int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
+               ffr_renderer **out)
+{
+    if (!flame_json || !out)
+        return fail(FFR_E_ARG, "ffr_create: null argument");
+    *out = nullptr;
+    ffr_options opt;
+    if (opt_in)
+        opt = *opt_in;
+    else
+        ffr_default_options(&opt);
+    if (opt.precision > FFR_FP32 || opt.word > FFR_WORD32)
+        return fail(FFR_E_ARG, "ffr_create: unknown precision or word option");
+    if (opt.precision == FFR_FP64 && opt.word == FFR_WORD32)
+        return fail(FFR_E_UNSUPPORTED,
+                    "FP64 trajectories need 64-bit buffer words (sizeof(num_t) == sizeof(hist_t))");
+    if (opt.bad_value_cap == 0)
+        opt.bad_value_cap = 4096;
+    std::unique_ptr<ffr_renderer> r(new ffr_renderer());
+    r->opt = opt;
+    r->device = opt.device;
+    r->mode = opt.precision == FFR_FP64 ? MODE_F64_U64
+                                        : (opt.word == FFR_WORD32 ? MODE_F32_U32 : MODE_F32_U64);
+    const bool fp32 = opt.precision == FFR_FP32;
+    try {
+        Json doc = Json::parse(std::string(flame_json, len));
+        r->spec = parse_flame(doc, fp32);
+        if (fp32)
+            r->f32 = compile_flame<float>(r->spec);
+        else
+            r->f64 = compile_flame<double>(r->spec);
+    } catch (const std::exception &e) {
+        return fail(FFR_E_FLAME, e.what());
+    }
+    HIP_TRY(hipSetDevice(r->device));
+    HIP_TRY(hipDeviceGetAttribute(&r->cu_count, hipDeviceAttributeMultiprocessorCount,
+                                  r->device));
+    int rc = fp32 ? setup_tables(r.get(), r->f32) : setup_tables(r.get(), r->f64);
+    if (rc != FFR_OK) {
+        ffr_destroy(r.release());
+        return rc;
+    }
+    r->cells = r->spec.cells();
+    r->cell_words = 1u + (uint32_t)r->spec.color_dims;
+    r->word_bytes = r->mode == MODE_F32_U32 ? 4u : 8u;
+    const size_t bytes = (size_t)r->cells * r->cell_words * r->word_bytes;
+    hipError_t e = hipMalloc(&r->dbuf, bytes);
+    if (e != hipSuccess) {
+        ffr_destroy(r.release());
+        return fail(FFR_E_DEVICE, "hipMalloc of the " + std::to_string(bytes) +
+                                      "-byte buffer: " + hipGetErrorString(e));
+    }
+    ffr_renderer *rp = r.release();
+#define HIP_TRY_D(call)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            ffr_destroy(rp);                                                           \
+            return fail(FFR_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                              \
+    } while (0)
+    HIP_TRY_D(hipStreamCreateWithFlags(&rp->own_stream, hipStreamNonBlocking));
+    rp->stream = rp->own_stream;
+    HIP_TRY_D(hipEventCreate(&rp->ev0));
+    HIP_TRY_D(hipEventCreate(&rp->ev1));
+    HIP_TRY_D(hipMalloc(&rp->dstats, sizeof(DevStats)));
+    HIP_TRY_D(hipMalloc(&rp->dxf, sizeof(unsigned long long) * (size_t)std::max(rp->nxf, 1)));
+    HIP_TRY_D(hipMalloc(&rp->dbv_xfs, sizeof(unsigned long long) * rp->opt.bad_value_cap));
+    HIP_TRY_D(hipMalloc(&rp->dbv_pts, sizeof(double) * 3 * rp->opt.bad_value_cap));
+    HIP_TRY_D(hipMalloc(&rp->dhist, sizeof(unsigned long long) * 3));
+    HIP_TRY_D((hipError_t)launch_fill_zero(rp->dbuf, bytes, rp->stream));
+    HIP_TRY_D(hipStreamSynchronize(rp->stream));
+#undef HIP_TRY_D
+    rp->xf_dist.assign(rp->spec.xforms.size(), 0);
+    for (int i = 0; i < 3; ++i) {
+        rp->pmin[i] = INFINITY;
+        rp->pmax[i] = -INFINITY;
+    }
+    *out = rp;
+    return FFR_OK;
+}
+
+void ffr_destroy(ffr_renderer *r)
+{
+    if (!r)
+        return;
+    // teardown: errors here have nowhere to go
+    (void)hipSetDevice(r->device);
+    if (r->own_stream)
+        (void)hipStreamSynchronize(r->own_stream);
+    if (r->stream && r->stream != r->own_stream)
+        (void)hipStreamSynchronize(r->stream);
+    (void)hipFree(r->tab.table);
+    (void)hipFree(r->tab.masks);
+    (void)hipFree(r->dbuf);
+    (void)hipFree(r->dstats);
+    (void)hipFree(r->dxf);
+    (void)hipFree(r->dbv_xfs);
+    (void)hipFree(r->dbv_pts);
+    (void)hipFree(r->dhist);
+    if (r->ev0)
+        (void)hipEventDestroy(r->ev0);
+    if (r->ev1)
+        (void)hipEventDestroy(r->ev1);
+    if (r->own_stream)
+        (void)hipStreamDestroy(r->own_stream);
+    delete r;
+}
+
+uint64_t ffr_auto_batch(const ffr_renderer *r, uint64_t samples)
+{
+    (void)r;
+    // enough walkers to fill the device (about 2^19 lanes), each long
+    // enough that its settle iterations stay a small share of its work
+    const uint64_t target_walkers = 1ull << 19;
+    uint64_t b = (samples + target_walkers - 1) / target_walkers;
+    if (b < 1024)
+        b = 1024;
+    return b;
+}
+
+} // extern "C"
+
+template <typename R>
+static void fill_args(ffr_renderer *r, const DeviceFlame<R> &d, WalkArgs<R> &a)
+{
+    std::memset(&a, 0, sizeof a);
+    a.table = (const R *)r->tab.table;
+    a.masks = r->tab.masks;
+    a.buf = r->dbuf;
+    a.stats = r->dstats;
+    a.xf_count = r->dxf;
+    a.bv_xfs = r->dbv_xfs;
+    a.bv_pts = r->dbv_pts;
+    for (int i = 0; i < 3; ++i) {
+        a.lo[i] = d.lo[i];
+        a.hi[i] = d.hi[i];
+        a.mult_d[i] = d.mult_d[i];
+        a.mult_i[i] = d.mult_i[i];
+    }
+    a.cells = d.cells;
+    a.seed = r->opt.seed;
+    a.bv_limit = r->opt.bad_value_limit;
+    a.bv_cap = r->opt.bad_value_cap;
+    a.nxf = d.nxf;
+    a.n_slots = (int)d.slot_type.size();
+    a.has_final = d.has_final;
+    a.fin_nv = (int)d.fin_type.size();
+    a.settle = r->opt.settle_iters ? (int)r->opt.settle_iters : (sizeof(R) == 4 ? 24 : 53);
+    a.r = d.r;
+    a.tab_reals = (int)d.table.size();
+    a.xf0 = d.xf0;
+    a.fin0 = d.fin0;
+    a.l_preA = d.lay.preA;
+    a.l_preb = d.lay.preb;
+    a.l_postA = d.lay.postA;
+    a.l_postb = d.lay.postb;
+    a.l_speed = d.lay.speed;
+    a.l_color = d.lay.color;
+    a.l_has_color = d.lay.has_color;
+    a.l_has_pre = d.lay.has_pre;
+    a.l_has_post = d.lay.has_post;
+    a.l_stride = d.lay.stride;
+    for (size_t s = 0; s < d.slot_type.size(); ++s) {
+        a.slot_type[s] = d.slot_type[s];
+        a.slot_off[s] = d.slot_off[s];
+        a.slot_stride[s] = d.slot_stride[s];
+    }
+    for (size_t j = 0; j < d.fin_type.size(); ++j) {
+        a.fin_type[j] = d.fin_type[j];
+        a.fin_off[j] = d.fin_off[j];
+    }
+}
+
+static int launch(ffr_renderer *r, uint64_t samples)
+{
+    uint64_t batch = r->opt.batch_size ? r->opt.batch_size : ffr_auto_batch(r, samples);
+    uint64_t walkers = (samples + batch - 1) / batch;
+    uint64_t last = samples - (walkers - 1) * batch;
+    const int block = walkers >= (uint64_t)r->cu_count * 256 ? 256 : 64;
+    uint64_t grid64 = (walkers + (uint64_t)block - 1) / (uint64_t)block;
+    const int grid = (int)std::min<uint64_t>(grid64, 1u << 30);
+    r->last_walkers = walkers;
+    r->last_batch = batch;
+    int rc = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream);
+    if (rc)
+        return fail(FFR_E_DEVICE, std::string("stats reset: ") + hipGetErrorString((hipError_t)rc));
+    HIP_TRY(hipEventRecord(r->ev0, r->stream));
+    const int dims = r->spec.dims, nc = r->spec.color_dims;
+    if (r->mode == MODE_F64_U64) {
+        WalkArgs<double> a;
+        fill_args(r, r->f64, a);
+        a.walker_base = r->next_walker;
+        a.n_walkers = walkers;
+        a.batch = batch;
+        a.last_samples = last;
+        rc = dims == 1 ? launch_walk_f64u64_d1(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
+           : dims == 2 ? launch_walk_f64u64_d2(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
+                       : launch_walk_f64u64_d3(nc, r->vs, a, grid, block, r->lds_bytes, r->stream);
+    } else {
+        WalkArgs<float> a;
+        fill_args(r, r->f32, a);
+        a.walker_base = r->next_walker;
+        a.n_walkers = walkers;
+        a.batch = batch;
+        a.last_samples = last;
+        if (r->mode == MODE_F32_U32)
+            rc = dims == 1 ? launch_walk_f32u32_d1(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
+               : dims == 2 ? launch_walk_f32u32_d2(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
+                           : launch_walk_f32u32_d3(nc, r->vs, a, grid, block, r->lds_bytes, r->stream);
+        else
+            rc = dims == 1 ? launch_walk_f32u64_d1(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
+               : dims == 2 ? launch_walk_f32u64_d2(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
+                           : launch_walk_f32u64_d3(nc, r->vs, a, grid, block, r->lds_bytes, r->stream);
+    }
+    if (rc)
+        return fail(FFR_E_DEVICE, std::string("walker kernel launch: ") +
+                                      hipGetErrorString((hipError_t)rc));
+    HIP_TRY(hipEventRecord(r->ev1, r->stream));
+    r->next_walker += walkers;
+    r->pending = true;
+    return FFR_OK;
+}
+
+static int finish(ffr_renderer *r, ffr_stats *out)
+{
+    if (r->pending) {
+        HIP_TRY(hipStreamSynchronize(r->stream));
+        r->pending = false;
+        DevStats st;
+        HIP_TRY(hipMemcpy(&st, r->dstats, sizeof st, hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> xf((size_t)std::max(r->nxf, 1));
+        HIP_TRY(hipMemcpy(xf.data(), r->dxf, sizeof(unsigned long long) * (size_t)r->nxf,
+                          hipMemcpyDeviceToHost));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+        r->last_ms = ms;
+        r->s_iter += st.s_iter;
+        r->s_plot += st.s_plot;
+        for (int k = 0; k < r->nxf; ++k)
+            r->xf_dist[(size_t)r->order[(size_t)k]] += xf[(size_t)k];
+        for (int i = 0; i < r->spec.dims; ++i) {
+            r->pmin[i] = std::min(r->pmin[i], st.pmin[i]);
+            r->pmax[i] = std::max(r->pmax[i], st.pmax[i]);
+        }
+        r->last_bad = st.n_bad;
+        size_t nrec = (size_t)std::min<uint64_t>(st.n_bad, r->opt.bad_value_cap);
+        r->bv_ids.assign(nrec, 0);
+        r->bv_pts.assign(nrec * 3, 0.0);
+        if (nrec) {
+            HIP_TRY(hipMemcpy(r->bv_ids.data(), r->dbv_xfs, sizeof(uint64_t) * nrec,
+                              hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(r->bv_pts.data(), r->dbv_pts, sizeof(double) * 3 * nrec,
+                              hipMemcpyDeviceToHost));
+            for (auto &id : r->bv_ids)
+                id = (uint64_t)r->order[(size_t)id];
+        }
+    }
+    if (out) {
+        std::memset(out, 0, sizeof *out);
+        out->s_iter = r->s_iter;
+        out->s_plot = r->s_plot;
+        out->n_bad = r->last_bad;
+        out->walkers = r->last_walkers;
+        out->batch_size = r->last_batch;
+        for (int i = 0; i < 3; ++i) {
+            out->pt_min[i] = i < r->spec.dims ? r->pmin[i] : 0.0;
+            out->pt_max[i] = i < r->spec.dims ? r->pmax[i] : 0.0;
+        }
+        out->kernel_ms = r->last_ms;
+    }
+    if (r->last_bad > r->opt.bad_value_limit)
+        return fail(FFR_E_BADVALUE_LIMIT, "render failure (flame may not be sufficiently contractive)");
+    return FFR_OK;
+}
+
+extern "C" {
+
+int ffr_render_async(ffr_renderer *r, uint64_t samples)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_render_async: null renderer");
+    if (r->pending)
+        return fail(FFR_E_ARG, "ffr_render_async: a render is already pending (call ffr_sync)");
+    HIP_TRY(hipSetDevice(r->device));
+    r->bv_ids.clear();
+    r->bv_pts.clear();
+    r->last_bad = 0;
+    if (samples == 0)
+        return FFR_OK;
+    return launch(r, samples);
+}
+
+int ffr_sync(ffr_renderer *r, ffr_stats *stats)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_sync: null renderer");
+    HIP_TRY(hipSetDevice(r->device));
+    return finish(r, stats);
+}
+
+int ffr_render(ffr_renderer *r, uint64_t samples, ffr_stats *stats)
+{
+    int rc = ffr_render_async(r, samples);
+    if (rc != FFR_OK)
+        return rc;
+    return ffr_sync(r, stats);
+}
+
+int ffr_set_stream(ffr_renderer *r, void *hip_stream)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_set_stream: null renderer");
+    if (r->pending)
+        return fail(FFR_E_ARG, "ffr_set_stream: a render is pending");
+    r->stream = hip_stream ? (hipStream_t)hip_stream : r->own_stream;
+    return FFR_OK;
+}
+
+void *ffr_device_buffer(ffr_renderer *r) { return r ? r->dbuf : nullptr; }
+uint64_t ffr_buffer_cells(const ffr_renderer *r) { return r ? r->cells : 0; }
+uint32_t ffr_cell_words(const ffr_renderer *r) { return r ? r->cell_words : 0; }
+uint32_t ffr_word_bytes(const ffr_renderer *r) { return r ? r->word_bytes : 0; }
+size_t ffr_buffer_bytes(const ffr_renderer *r)
+{
+    return r ? (size_t)r->cells * r->cell_words * r->word_bytes : 0;
+}
+int32_t ffr_dims(const ffr_renderer *r) { return r ? r->spec.dims : 0; }
+int32_t ffr_color_dims(const ffr_renderer *r) { return r ? r->spec.color_dims : 0; }
+
+int ffr_read_buffer(ffr_renderer *r, void *dst, size_t bytes)
+{
+    if (!r || !dst)
+        return fail(FFR_E_ARG, "ffr_read_buffer: null argument");
+    if (bytes != ffr_buffer_bytes(r))
+        return fail(FFR_E_SIZE, "ffr_read_buffer: size mismatch");
+    HIP_TRY(hipSetDevice(r->device));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    HIP_TRY(hipMemcpy(dst, r->dbuf, bytes, hipMemcpyDeviceToHost));
+    return FFR_OK;
+}
+
+int ffr_write_buffer(ffr_renderer *r, const void *src, size_t bytes)
+{
+    if (!r || !src)
+        return fail(FFR_E_ARG, "ffr_write_buffer: null argument");
+    if (bytes != ffr_buffer_bytes(r))
+        return fail(FFR_E_SIZE, "ffr_write_buffer: size mismatch");
+    HIP_TRY(hipSetDevice(r->device));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    HIP_TRY(hipMemcpy(r->dbuf, src, bytes, hipMemcpyHostToDevice));
+    return FFR_OK;
+}
+
+int ffr_add_buffer(ffr_renderer *r, const void *src, size_t bytes)
+{
+    if (!r || !src)
+        return fail(FFR_E_ARG, "ffr_add_buffer: null argument");
+    if (bytes != ffr_buffer_bytes(r))
+        return fail(FFR_E_SIZE, "BufferRenderer::addBuffer(): sizes do not match");
+    HIP_TRY(hipSetDevice(r->device));
+    // staged in chunks of whole cells through one device scratch buffer
+    const size_t cell_bytes = (size_t)r->cell_words * r->word_bytes;
+    const size_t chunk_cells = std::max<size_t>(1, (size_t)(256u << 20) / cell_bytes);
+    const size_t chunk = std::min(bytes, chunk_cells * cell_bytes);
+    void *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, chunk));
+    int rc = FFR_OK;
+    for (size_t off = 0; off < bytes && rc == FFR_OK; off += chunk) {
+        size_t n = std::min(chunk, bytes - off);
+        hipError_t e = hipMemcpyAsync(tmp, (const char *)src + off, n, hipMemcpyHostToDevice,
+                                      r->stream);
+        if (e == hipSuccess)
+            e = (hipError_t)launch_add_cells((char *)r->dbuf + off, tmp,
+                                             n / r->word_bytes, (int)r->cell_words,
+                                             (int)r->word_bytes, r->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess)
+            rc = fail(FFR_E_DEVICE, std::string("ffr_add_buffer: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(tmp);
+    return rc;
+}
+
+int ffr_check_flame(const char *flame_json, size_t len, uint32_t precision, int32_t *ids,
+                    double *cw, size_t cap, size_t *n_xforms)
+{
+    if (!flame_json)
+        return fail(FFR_E_ARG, "ffr_check_flame: null flame");
+    try {
+        Json doc = Json::parse(std::string(flame_json, len));
+        FlameSpec spec = parse_flame(doc, precision == FFR_FP32);
+        std::vector<int> order;
+        std::vector<double> cum;
+        if (precision == FFR_FP32) {
+            auto d = compile_flame<float>(spec);
+            order = d.order;
+            for (float v : d.cw)
+                cum.push_back((double)v);
+        } else {
+            auto d = compile_flame<double>(spec);
+            order = d.order;
+            cum = d.cw;
+        }
+        for (size_t k = 0; k < order.size() && k < cap; ++k) {
+            if (ids)
+                ids[k] = order[k];
+            if (cw)
+                cw[k] = cum[k];
+        }
+        if (n_xforms)
+            *n_xforms = order.size();
+    } catch (const std::exception &e) {
+        return fail(FFR_E_FLAME, e.what());
+    }
+    return FFR_OK;
+}
+
+int ffr_clear_buffer(ffr_renderer *r)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_clear_buffer: null renderer");
+    HIP_TRY(hipSetDevice(r->device));
+    HIP_TRY((hipError_t)launch_fill_zero(r->dbuf, ffr_buffer_bytes(r), r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return FFR_OK;
+}
+
+int ffr_histogram_stats(ffr_renderer *r, uint64_t *sum, uint64_t *mn, uint64_t *mx)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_histogram_stats: null renderer");
+    HIP_TRY(hipSetDevice(r->device));
+    HIP_TRY((hipError_t)launch_hist_stats(r->dbuf, r->cells, (int)r->cell_words,
+                                          (int)r->word_bytes, r->dhist, r->stream));
+    unsigned long long h[3];
+    HIP_TRY(hipMemcpyAsync(h, r->dhist, sizeof h, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    if (sum)
+        *sum = h[0];
+    if (mn)
+        *mn = (r->word_bytes == 4 && h[1] == ~0ull) ? 0xffffffffull : h[1];
+    if (mx)
+        *mx = h[2];
+    return FFR_OK;
+}
+
+size_t ffr_xform_id_count(const ffr_renderer *r) { return r ? r->xf_dist.size() : 0; }
+
+int ffr_xform_distribution(const ffr_renderer *r, uint64_t *out, size_t n)
+{
+    if (!r || (!out && n))
+        return fail(FFR_E_ARG, "ffr_xform_distribution: null argument");
+    if (n != r->xf_dist.size())
+        return fail(FFR_E_SIZE, "ffr_xform_distribution: expected " +
+                                    std::to_string(r->xf_dist.size()) + " entries");
+    std::copy(r->xf_dist.begin(), r->xf_dist.end(), out);
+    return FFR_OK;
+}
+
+int ffr_bad_values(const ffr_renderer *r, uint64_t *ids, double *pts, size_t cap, size_t *n)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_bad_values: null renderer");
+    size_t m = std::min(cap, r->bv_ids.size());
+    const int d = r->spec.dims;
+    for (size_t i = 0; i < m; ++i) {
+        if (ids)
+            ids[i] = r->bv_ids[i];
+        if (pts)
+            for (int k = 0; k < d; ++k)
+                pts[i * (size_t)d + (size_t)k] = r->bv_pts[i * 3 + (size_t)k];
+    }
+    if (n)
+        *n = r->bv_ids.size();
+    return FFR_OK;
+}
+
+int ffr_xform_order(const ffr_renderer *r, int32_t *ids, double *cw, size_t n)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_xform_order: null renderer");
+    if (n != (size_t)r->nxf)
+        return fail(FFR_E_SIZE, "ffr_xform_order: expected " + std::to_string(r->nxf) +
+                                    " entries");
+    for (int k = 0; k < r->nxf; ++k) {
+        if (ids)
+            ids[k] = r->order[(size_t)k];
+        if (cw)
+            cw[k] = r->mode == MODE_F64_U64 ? r->f64.cw[(size_t)k] : (double)r->f32.cw[(size_t)k];
+    }
+    return FFR_OK;
+}
+
+} // extern "C"

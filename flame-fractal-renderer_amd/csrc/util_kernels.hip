@@ -1,0 +1,134 @@
+// Buffer and statistics kernels used by the C ABI.
+#include "util_kernels.hpp"
+
+namespace ffr {
+
+__global__ void reset_stats_kernel(DevStats *st, unsigned long long *xf_count, int nxf)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        st->s_iter = 0;
+        st->s_plot = 0;
+        st->n_bad = 0;
+        st->pad = 0;
+        for (int i = 0; i < 3; ++i) {
+            st->pmin[i] = (double)INFINITY;
+            st->pmax[i] = -(double)INFINITY;
+        }
+    }
+    for (int k = t; k < nxf; k += gridDim.x * blockDim.x)
+        xf_count[k] = 0;
+}
+
+// addBuffer (buffer_renderer.hpp:375-392): word 0 of a cell adds as an integer,
+// the colour words add as floating point
+template <typename CT, typename CR>
+__global__ void add_cells_kernel(CT *dst, const CT *src, unsigned long long words, int cw)
+{
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         i < words; i += (unsigned long long)gridDim.x * blockDim.x) {
+        if (i % (unsigned long long)cw == 0) {
+            dst[i] += src[i];
+        } else {
+            CR *d = reinterpret_cast<CR *>(dst + i);
+            const CR *s = reinterpret_cast<const CR *>(src + i);
+            *d = *d + *s;
+        }
+    }
+}
+
+template <typename CT>
+__global__ void hist_stats_kernel(const CT *buf, unsigned long long cells, int cw,
+                                  unsigned long long *out /* sum, min, max */)
+{
+    unsigned long long sum = 0, mn = ~0ull, mx = 0;
+    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         c < cells; c += (unsigned long long)gridDim.x * blockDim.x) {
+        unsigned long long v = (unsigned long long)buf[c * (unsigned long long)cw];
+        sum += v;
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off, 64);
+        unsigned long long a = __shfl_xor(mn, off, 64);
+        unsigned long long b = __shfl_xor(mx, off, 64);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], sum);
+        atomicMin(&out[1], mn);
+        atomicMax(&out[2], mx);
+    }
+}
+
+__global__ void hist_stats_init_kernel(unsigned long long *out)
+{
+    out[0] = 0;
+    out[1] = ~0ull;
+    out[2] = 0;
+}
+
+__global__ void fill_zero_kernel(unsigned int *p, unsigned long long n)
+{
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+static int grid_for(unsigned long long n, int block)
+{
+    unsigned long long g = (n + (unsigned long long)block - 1) / (unsigned long long)block;
+    if (g > 65536ull)
+        g = 65536ull;
+    return g ? (int)g : 1;
+}
+
+int launch_reset_stats(DevStats *st, unsigned long long *xf_count, int nxf, hipStream_t s)
+{
+    hipLaunchKernelGGL(reset_stats_kernel, dim3(1), dim3(256), 0, s, st, xf_count, nxf);
+    return (int)hipGetLastError();
+}
+
+int launch_add_cells(void *dst, const void *src, unsigned long long words, int cw,
+                     int word_bytes, hipStream_t s)
+{
+    const int block = 256;
+    const int grid = grid_for(words, block);
+    if (word_bytes == 8)
+        hipLaunchKernelGGL((add_cells_kernel<unsigned long long, double>), dim3(grid),
+                           dim3(block), 0, s, (unsigned long long *)dst,
+                           (const unsigned long long *)src, words, cw);
+    else
+        hipLaunchKernelGGL((add_cells_kernel<unsigned int, float>), dim3(grid), dim3(block), 0,
+                           s, (unsigned int *)dst, (const unsigned int *)src, words, cw);
+    return (int)hipGetLastError();
+}
+
+int launch_hist_stats(const void *buf, unsigned long long cells, int cw, int word_bytes,
+                      unsigned long long *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(hist_stats_init_kernel, dim3(1), dim3(1), 0, s, out);
+    const int block = 256;
+    const int grid = grid_for(cells, block);
+    if (word_bytes == 8)
+        hipLaunchKernelGGL((hist_stats_kernel<unsigned long long>), dim3(grid), dim3(block), 0,
+                           s, (const unsigned long long *)buf, cells, cw, out);
+    else
+        hipLaunchKernelGGL((hist_stats_kernel<unsigned int>), dim3(grid), dim3(block), 0, s,
+                           (const unsigned int *)buf, cells, cw, out);
+    return (int)hipGetLastError();
+}
+
+int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s)
+{
+    // buffers are whole words of 4 or 8 bytes
+    const unsigned long long n = bytes / 4;
+    const int block = 256;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(n, block)), dim3(block), 0, s,
+                       (unsigned int *)p, n);
+    return (int)hipGetLastError();
+}
+
+} // namespace ffr

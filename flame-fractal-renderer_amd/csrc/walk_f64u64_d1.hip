@@ -1,0 +1,10 @@
+// Walker kernels for 1-D flames, num_t = double, hist_t = uint64 (the reference's default build).
+#include "walk_kernel.inc"
+
+namespace ffr {
+int launch_walk_f64u64_d1(int nc, int vs, const WalkArgs<double> &a, int grid, int block,
+                          size_t lds, hipStream_t s)
+{
+    return launch_nc<double, unsigned long long, double, 1>(nc, vs, a, grid, block, lds, s);
+}
+} // namespace ffr

@@ -1,0 +1,303 @@
+"""Python binding of libffr_hip (the MI355X flame-buffer renderer).
+
+`BufferRenderer` mirrors the reference's C++ class of the same name
+(src/renderers/buffer_renderer.hpp:38-585): construct from a flame JSON,
+`render(samples)`, `add_buffer`, `read_buffer`/`write_buffer`, and the
+statistics getters.  The work runs in the HIP kernels of libffr_hip.so; this
+module only marshals arguments over the C ABI (include/ffr.h).  There is no
+CPU fallback: if the library is missing, loading fails with an error.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffr_hip.so")
+CLI_PATH = os.path.join(_HERE, "ffr-buf")
+
+FFR_OK, FFR_E_ARG, FFR_E_FLAME, FFR_E_DEVICE = 0, 1, 2, 3
+FFR_E_BADVALUE_LIMIT, FFR_E_SIZE, FFR_E_UNSUPPORTED = 4, 5, 6
+FFR_FP64, FFR_FP32 = 0, 1
+FFR_WORD64, FFR_WORD32 = 0, 1
+
+
+class FfrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_uint32),
+        ("word", C.c_uint32),
+        ("settle_iters", C.c_uint32),
+        ("device", C.c_int32),
+        ("seed", C.c_uint64),
+        ("batch_size", C.c_uint64),
+        ("bad_value_limit", C.c_uint64),
+        ("bad_value_cap", C.c_uint64),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("s_iter", C.c_uint64),
+        ("s_plot", C.c_uint64),
+        ("n_bad", C.c_uint64),
+        ("walkers", C.c_uint64),
+        ("batch_size", C.c_uint64),
+        ("pt_min", C.c_double * 3),
+        ("pt_max", C.c_double * 3),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+# every symbol include/ffr.h declares
+EXPORTS = [
+    "ffr_default_options", "ffr_create", "ffr_destroy", "ffr_last_error",
+    "ffr_render", "ffr_render_async", "ffr_sync", "ffr_set_stream",
+    "ffr_device_buffer", "ffr_buffer_cells", "ffr_cell_words", "ffr_word_bytes",
+    "ffr_buffer_bytes", "ffr_dims", "ffr_color_dims", "ffr_read_buffer",
+    "ffr_add_buffer", "ffr_write_buffer", "ffr_clear_buffer",
+    "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
+    "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libffr_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libffr_hip.so is not built ({LIB_PATH}); run "
+            "`make -C flame-fractal-renderer_amd` or __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    L.ffr_default_options.argtypes = [C.POINTER(Options)]
+    L.ffr_create.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(Options), C.POINTER(P)]
+    L.ffr_destroy.argtypes = [P]
+    L.ffr_last_error.restype = C.c_char_p
+    L.ffr_render.argtypes = [P, C.c_uint64, C.POINTER(Stats)]
+    L.ffr_render_async.argtypes = [P, C.c_uint64]
+    L.ffr_sync.argtypes = [P, C.POINTER(Stats)]
+    L.ffr_set_stream.argtypes = [P, P]
+    L.ffr_device_buffer.restype = P
+    L.ffr_device_buffer.argtypes = [P]
+    L.ffr_buffer_cells.restype = C.c_uint64
+    L.ffr_buffer_cells.argtypes = [P]
+    L.ffr_cell_words.restype = C.c_uint32
+    L.ffr_cell_words.argtypes = [P]
+    L.ffr_word_bytes.restype = C.c_uint32
+    L.ffr_word_bytes.argtypes = [P]
+    L.ffr_buffer_bytes.restype = C.c_size_t
+    L.ffr_buffer_bytes.argtypes = [P]
+    L.ffr_dims.restype = C.c_int32
+    L.ffr_dims.argtypes = [P]
+    L.ffr_color_dims.restype = C.c_int32
+    L.ffr_color_dims.argtypes = [P]
+    for name in ("ffr_read_buffer", "ffr_add_buffer", "ffr_write_buffer"):
+        getattr(L, name).argtypes = [P, P, C.c_size_t]
+    L.ffr_clear_buffer.argtypes = [P]
+    L.ffr_histogram_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]
+    L.ffr_xform_distribution.argtypes = [P, C.POINTER(C.c_uint64), C.c_size_t]
+    L.ffr_xform_id_count.restype = C.c_size_t
+    L.ffr_xform_id_count.argtypes = [P]
+    L.ffr_bad_values.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                 C.c_size_t, C.POINTER(C.c_size_t)]
+    L.ffr_xform_order.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                  C.c_size_t]
+    L.ffr_auto_batch.restype = C.c_uint64
+    L.ffr_auto_batch.argtypes = [P, C.c_uint64]
+    L.ffr_check_flame.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32,
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                  C.c_size_t, C.POINTER(C.c_size_t)]
+    for name in EXPORTS:
+        if not hasattr(L, name):
+            raise RuntimeError(f"libffr_hip.so does not export {name}")
+    # functions not given a restype above return int (the ctypes default)
+    L.ffr_default_options.restype = None
+    L.ffr_destroy.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != FFR_OK:
+        raise FfrError(rc, lib().ffr_last_error().decode(errors="replace"))
+
+
+def check_flame(text: str, precision: str = "fp64"):
+    """Validate a flame without a device; returns (xform ids, cum weights)."""
+    L = lib()
+    b = text.encode()
+    n = C.c_size_t(0)
+    cap = 4096
+    ids = (C.c_int32 * cap)()
+    cw = (C.c_double * cap)()
+    prec = FFR_FP32 if precision == "fp32" else FFR_FP64
+    _check(L.ffr_check_flame(b, len(b), prec, ids, cw, cap, C.byref(n)))
+    return [ids[i] for i in range(n.value)], [cw[i] for i in range(n.value)]
+
+
+class BufferRenderer:
+    """A flame and its device buffer (reference BufferRenderer<dims>)."""
+
+    def __init__(self, flame_json: str, precision: str = "fp64", word: int = 64,
+                 seed: int = 0, batch_size: int = 0, bad_value_limit: int = 256,
+                 settle: int = 0, device: int = 0, bad_value_cap: int = 4096):
+        L = lib()
+        o = Options()
+        L.ffr_default_options(C.byref(o))
+        o.precision = FFR_FP32 if precision == "fp32" else FFR_FP64
+        o.word = FFR_WORD32 if word == 32 else FFR_WORD64
+        o.settle_iters = settle
+        o.device = device
+        o.seed = seed
+        o.batch_size = batch_size
+        o.bad_value_limit = bad_value_limit
+        o.bad_value_cap = bad_value_cap
+        h = C.c_void_p()
+        b = flame_json.encode()
+        _check(L.ffr_create(b, len(b), C.byref(o), C.byref(h)))
+        self._h = h
+        self.precision = precision
+        self.dims = L.ffr_dims(h)
+        self.color_dims = L.ffr_color_dims(h)
+        self.cells = L.ffr_buffer_cells(h)
+        self.cell_words = L.ffr_cell_words(h)
+        self.word_bytes = L.ffr_word_bytes(h)
+        self.last_stats = Stats()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ffr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- rendering (buffer_renderer.hpp:269-372) ---
+    def render(self, samples: int) -> bool:
+        """Render; False when the bad-value limit stopped it."""
+        rc = lib().ffr_render(self._h, samples, C.byref(self.last_stats))
+        if rc == FFR_E_BADVALUE_LIMIT:
+            return False
+        _check(rc)
+        return True
+
+    def render_async(self, samples: int):
+        _check(lib().ffr_render_async(self._h, samples))
+
+    def sync(self) -> bool:
+        rc = lib().ffr_sync(self._h, C.byref(self.last_stats))
+        if rc == FFR_E_BADVALUE_LIMIT:
+            return False
+        _check(rc)
+        return True
+
+    def set_stream(self, stream_ptr: int | None):
+        _check(lib().ffr_set_stream(self._h, C.c_void_p(stream_ptr or 0)))
+
+    def device_buffer(self) -> int:
+        return lib().ffr_device_buffer(self._h)
+
+    def auto_batch(self, samples: int) -> int:
+        return lib().ffr_auto_batch(self._h, samples)
+
+    # --- buffer (buffer_renderer.hpp:375-480) ---
+    @property
+    def buffer_bytes(self) -> int:
+        return lib().ffr_buffer_bytes(self._h)
+
+    def _dtype(self):
+        return np.uint32 if self.word_bytes == 4 else np.uint64
+
+    def read_buffer(self) -> np.ndarray:
+        out = np.empty(self.cells * self.cell_words, dtype=self._dtype())
+        _check(lib().ffr_read_buffer(self._h, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def write_buffer(self, words: np.ndarray):
+        w = np.ascontiguousarray(words, dtype=self._dtype())
+        _check(lib().ffr_write_buffer(self._h, w.ctypes.data_as(C.c_void_p), w.nbytes))
+
+    def add_buffer(self, words: np.ndarray):
+        w = np.ascontiguousarray(words, dtype=self._dtype())
+        _check(lib().ffr_add_buffer(self._h, w.ctypes.data_as(C.c_void_p), w.nbytes))
+
+    def clear_buffer(self):
+        _check(lib().ffr_clear_buffer(self._h))
+
+    def counts(self, words: np.ndarray | None = None) -> np.ndarray:
+        w = self.read_buffer() if words is None else words
+        return w.reshape(self.cells, self.cell_words)[:, 0].copy()
+
+    def colors(self, words: np.ndarray | None = None) -> np.ndarray:
+        w = self.read_buffer() if words is None else words
+        cols = w.reshape(self.cells, self.cell_words)[:, 1:].copy()
+        return cols.view(np.float32 if self.word_bytes == 4 else np.float64)
+
+    def histogram_stats(self):
+        s, mn, mx = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().ffr_histogram_stats(self._h, C.byref(s), C.byref(mn), C.byref(mx)))
+        return s.value, mn.value, mx.value
+
+    # --- statistics getters (buffer_renderer.hpp:534-564) ---
+    @property
+    def samples_iterated(self) -> int:
+        return self.last_stats.s_iter
+
+    @property
+    def samples_plotted(self) -> int:
+        return self.last_stats.s_plot
+
+    @property
+    def point_extremes(self):
+        return [(self.last_stats.pt_min[i], self.last_stats.pt_max[i])
+                for i in range(self.dims)]
+
+    @property
+    def kernel_ms(self) -> float:
+        return self.last_stats.kernel_ms
+
+    def xform_distribution(self) -> np.ndarray:
+        n = lib().ffr_xform_id_count(self._h)
+        out = (C.c_uint64 * max(1, n))()
+        _check(lib().ffr_xform_distribution(self._h, out, n))
+        return np.array([out[i] for i in range(n)], dtype=np.uint64)
+
+    def bad_values(self):
+        n = C.c_size_t(0)
+        lib().ffr_bad_values(self._h, None, None, 0, C.byref(n))
+        k = n.value
+        ids = (C.c_uint64 * max(1, k))()
+        pts = (C.c_double * max(1, k * self.dims))()
+        _check(lib().ffr_bad_values(self._h, ids, pts, k, C.byref(n)))
+        return (np.array([ids[i] for i in range(k)], dtype=np.uint64),
+                np.array([pts[i] for i in range(k * self.dims)]).reshape(k, self.dims))
+
+    def xform_order(self):
+        n = len(self.xform_distribution())
+        ids = (C.c_int32 * max(1, n))()
+        cw = (C.c_double * max(1, n))()
+        # the number of xforms kept is at most the number in the file
+        for m in range(n, 0, -1):
+            if lib().ffr_xform_order(self._h, ids, cw, m) == FFR_OK:
+                return [ids[i] for i in range(m)], [cw[i] for i in range(m)]
+        raise FfrError(FFR_E_SIZE, "no xforms")

@@ -1,0 +1,148 @@
+/*
+ * libffr_hip -- MI355X (gfx950) flame-buffer renderer, C ABI.
+ *
+ * This is the drop-in boundary for the reference's buffer-render path
+ * (tkoz0/flame-fractal-renderer).  The reference has no C ABI: its interface
+ * is the C++ class BufferRenderer<dims> (src/renderers/buffer_renderer.hpp)
+ * built from a Flame<dims> parsed from JSON (src/types/flame.hpp), driven by
+ * the ffr-buf program (src/ffr_buf.cpp).  Each entry point below names the
+ * reference member it replaces.  No torch or HIP types appear in the
+ * signatures; device streams are passed as opaque pointers.
+ *
+ * Conventions: functions return FFR_OK (0) or an FFR_E_* code and leave a
+ * message for ffr_last_error() (thread-local).  The caller owns host memory,
+ * the renderer owns device memory.  One renderer must not be used from two
+ * host threads at once (the reference object is not re-entrant either).
+ */
+#ifndef FFR_H
+#define FFR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ffr_renderer ffr_renderer;
+
+enum {
+    FFR_OK = 0,
+    FFR_E_ARG = 1,              /* invalid argument (reference: runtime_error) */
+    FFR_E_FLAME = 2,            /* flame JSON rejected (reference: JsonError)  */
+    FFR_E_DEVICE = 3,           /* HIP runtime failure                         */
+    FFR_E_BADVALUE_LIMIT = 4,   /* render stopped: render() returned false     */
+    FFR_E_SIZE = 5,             /* buffer size mismatch (addBuffer/readBuffer) */
+    FFR_E_UNSUPPORTED = 6       /* flame outside this build's limits           */
+};
+
+/* trajectory precision: num_t of the reference (src/types/types.hpp:29) */
+enum { FFR_FP64 = 0, FFR_FP32 = 1 };
+/* buffer word: hist_t of the reference (src/types/types.hpp:35) */
+enum { FFR_WORD64 = 0, /* uint64 count + double colour sums (default build) */
+       FFR_WORD32 = 1  /* uint32 count + float colour sums (float build)    */ };
+
+typedef struct ffr_options {
+    uint32_t precision;      /* FFR_FP64 or FFR_FP32                                  */
+    uint32_t word;           /* FFR_WORD64 or FFR_WORD32; FP64 needs FFR_WORD64       */
+    uint32_t settle_iters;   /* 0: reference default, 53 (FP64) / 24 (FP32)           */
+    int32_t device;          /* HIP device ordinal                                    */
+    uint64_t seed;           /* key of every walker's random stream                   */
+    uint64_t batch_size;     /* samples per walker (ffr-buf -b); 0 = chosen for the GPU */
+    uint64_t bad_value_limit;/* ffr-buf -z (default 256)                              */
+    uint64_t bad_value_cap;  /* bad-value records kept per render (default 4096)      */
+} ffr_options;
+
+typedef struct ffr_stats {
+    uint64_t s_iter;          /* getSamplesIterated (cumulative)                       */
+    uint64_t s_plot;          /* getSamplesPlotted (cumulative)                        */
+    uint64_t n_bad;           /* bad values seen by the last render                    */
+    uint64_t walkers;         /* walkers (reference batches) of the last render        */
+    uint64_t batch_size;      /* samples per walker of the last render                 */
+    double pt_min[3];         /* getPointExtremes (cumulative)                         */
+    double pt_max[3];
+    double kernel_ms;         /* device time of the last render's kernel (HIP events)  */
+} ffr_stats;
+
+/* defaults: FP64, WORD64, settle 0, device 0, seed 0, batch 0, limit 256 */
+void ffr_default_options(ffr_options *opt);
+
+/*
+ * Parse and validate a flame, allocate and zero the buffer on the device.
+ * Replaces Flame<dims>(const Json&) (flame.hpp:91-210) followed by
+ * BufferRenderer<dims>(flame) (buffer_renderer.hpp:254, _init :114-140).
+ * On FFR_E_FLAME the message is the reference's JsonError text.
+ */
+int ffr_create(const char *flame_json, size_t len, const ffr_options *opt,
+               ffr_renderer **out);
+void ffr_destroy(ffr_renderer *r);
+const char *ffr_last_error(void);
+
+/*
+ * Render `samples` samples into the buffer and wait for them.  Replaces
+ * BufferRenderer::render / renderSeeded (buffer_renderer.hpp:269-372):
+ * walkers take consecutive stream indices across calls, each renders
+ * batch_size samples (the last one the rest).  Returns FFR_E_BADVALUE_LIMIT
+ * when more than bad_value_limit bad values stopped the render (render()
+ * returning false).  stats may be NULL.
+ */
+int ffr_render(ffr_renderer *r, uint64_t samples, ffr_stats *stats);
+
+/* enqueue the same work on the renderer's stream without waiting;
+ * ffr_sync waits, folds the statistics in and reports like ffr_render */
+int ffr_render_async(ffr_renderer *r, uint64_t samples);
+int ffr_sync(ffr_renderer *r, ffr_stats *stats);
+
+/* launch on a caller-owned hipStream_t (NULL: the renderer's own stream) */
+int ffr_set_stream(ffr_renderer *r, void *hip_stream);
+/* device address of the buffer (cells * cell_words words) */
+void *ffr_device_buffer(ffr_renderer *r);
+
+/* buffer geometry: getBufferNumCells / getBufferCellSize (:524-532) */
+uint64_t ffr_buffer_cells(const ffr_renderer *r);
+uint32_t ffr_cell_words(const ffr_renderer *r);
+uint32_t ffr_word_bytes(const ffr_renderer *r);
+size_t ffr_buffer_bytes(const ffr_renderer *r);
+int32_t ffr_dims(const ffr_renderer *r);
+int32_t ffr_color_dims(const ffr_renderer *r);
+
+/* writeBuffer (:476-480): the raw buffer, [count, colour_0..] per cell,
+ * dimension 0 fastest, native endian; bytes must equal ffr_buffer_bytes */
+int ffr_read_buffer(ffr_renderer *r, void *dst, size_t bytes);
+/* addBuffer (:375-452): add a buffer of the same layout cell by cell
+ * (counts as integers, colours as floats); bytes must equal ffr_buffer_bytes */
+int ffr_add_buffer(ffr_renderer *r, const void *src, size_t bytes);
+/* readBuffer (:457-473): overwrite the buffer */
+int ffr_write_buffer(ffr_renderer *r, const void *src, size_t bytes);
+int ffr_clear_buffer(ffr_renderer *r);
+
+/* histogramSum / histogramMin / histogramMax (:483-507), on the device */
+int ffr_histogram_stats(ffr_renderer *r, uint64_t *sum, uint64_t *min, uint64_t *max);
+
+/* getXFormDistribution (:544-548): selections per xform id (file order) */
+int ffr_xform_distribution(const ffr_renderer *r, uint64_t *out, size_t n);
+size_t ffr_xform_id_count(const ffr_renderer *r);
+/* getBadValueXForms / getBadValuePoints (:550-558) of the last render;
+ * points are dims doubles each; *n receives the number recorded */
+int ffr_bad_values(const ffr_renderer *r, uint64_t *xform_ids, double *points,
+                   size_t cap, size_t *n);
+/* xform ids in selection order (after Flame::_optimize, flame.hpp:63-83)
+ * and their cumulative weights as doubles */
+int ffr_xform_order(const ffr_renderer *r, int32_t *ids, double *cum_weights, size_t n);
+
+/* the batch size the renderer picks for `samples` when batch_size is 0 */
+uint64_t ffr_auto_batch(const ffr_renderer *r, uint64_t samples);
+
+/*
+ * Validate a flame without touching a device: Flame<dims>(const Json&) and
+ * Flame::_optimize (flame.hpp:63-210).  On success *n_xforms receives the
+ * number of xforms left; up to cap of their ids (selection order) and
+ * cumulative weights are written when the arrays are not NULL.
+ */
+int ffr_check_flame(const char *flame_json, size_t len, uint32_t precision, int32_t *ids,
+                    double *cum_weights, size_t cap, size_t *n_xforms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
