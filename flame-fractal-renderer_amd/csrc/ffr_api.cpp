@@ -9,9 +9,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -81,6 +83,14 @@ struct ffr_renderer {
     bool pending = false;
     int nxf = 0;
     std::vector<int> order; // sorted position -> xform id
+    // launch policy: LDS cell cache (-1 auto, 0 off, 1 on), its LDS budget in
+    // KiB, block size (0 auto); FFR_CACHE / FFR_CACHE_KB / FFR_BLOCK override
+    int cache_pref = -1;
+    int cache_kb = 144;
+    int block_pref = 0;
+    size_t max_lds = 65536;
+    size_t lds_per_cu = 65536;
+    int last_cache = 0;
 };
 
 static int pick_variation_set(const uint32_t ts[4])
@@ -138,6 +148,7 @@ void ffr_default_options(ffr_options *opt)
     std::memset(opt, 0, sizeof *opt);
     opt->precision = FFR_FP64;
     opt->word = FFR_WORD64;
+    opt->settle_iters = -1;
     opt->bad_value_limit = 256;
     opt->bad_value_cap = 4096;
 }
@@ -181,6 +192,23 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
     HIP_TRY(hipSetDevice(r->device));
     HIP_TRY(hipDeviceGetAttribute(&r->cu_count, hipDeviceAttributeMultiprocessorCount,
                                   r->device));
+    {
+        int v = 0;
+        HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, r->device));
+        r->max_lds = (size_t)v;
+        v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  r->device) == hipSuccess && v > 0)
+            r->lds_per_cu = (size_t)v;
+        else
+            r->lds_per_cu = r->max_lds;
+        if (const char *e = std::getenv("FFR_CACHE"))
+            r->cache_pref = std::atoi(e);
+        if (const char *e = std::getenv("FFR_CACHE_KB"))
+            r->cache_kb = std::max(1, std::atoi(e));
+        if (const char *e = std::getenv("FFR_BLOCK"))
+            r->block_pref = std::atoi(e);
+    }
     int rc = fp32 ? setup_tables(r.get(), r->f32) : setup_tables(r.get(), r->f64);
     if (rc != FFR_OK) {
         ffr_destroy(r.release());
@@ -292,7 +320,7 @@ static void fill_args(ffr_renderer *r, const DeviceFlame<R> &d, WalkArgs<R> &a)
     a.n_slots = (int)d.slot_type.size();
     a.has_final = d.has_final;
     a.fin_nv = (int)d.fin_type.size();
-    a.settle = r->opt.settle_iters ? (int)r->opt.settle_iters : (sizeof(R) == 4 ? 24 : 53);
+    a.settle = r->opt.settle_iters >= 0 ? (int)r->opt.settle_iters : (sizeof(R) == 4 ? 24 : 53);
     a.r = d.r;
     a.tab_reals = (int)d.table.size();
     a.xf0 = d.xf0;
@@ -318,50 +346,135 @@ static void fill_args(ffr_renderer *r, const DeviceFlame<R> &d, WalkArgs<R> &a)
     }
 }
 
+// launch geometry of one render pass (see plan_launch)
+struct LaunchPlan {
+    bool cache = false;
+    int block = 256;
+    int grid = 1;
+    size_t lds = 0;
+    int slots = 0, log2s = 0, ways = 1, direct = 0;
+    uint64_t walkers_per_launch = 0;
+};
+
+static size_t table_lds(const ffr_renderer *r)
+{
+    const int rb = r->mode == MODE_F64_U64 ? 8 : 4;
+    return cache_lds_offset(r->tab.tab_reals, rb, r->nxf);
+}
+
+// Accumulation through the LDS cell cache needs cell indices below 2^32,
+// at most 4 colour words, and per-workgroup counts that fit 32 bits.  The
+// cache takes the LDS left after the flame table, up to the budget; a
+// buffer that fits entirely is indexed directly.
+static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t batch)
+{
+    LaunchPlan p;
+    const size_t base = table_lds(r);
+    const int colour_bytes = r->word_bytes;
+    const size_t slot_bytes = cache_slot_bytes(r->spec.color_dims, colour_bytes);
+    bool want = r->cache_pref != 0 && r->cells < 0xFFFFFFFFull && r->spec.color_dims <= 4;
+    p.block = r->block_pref > 0 ? r->block_pref : 512;
+    if (want && (uint64_t)p.block * batch >= (1ull << 31))
+        want = false;
+    if (want) {
+        size_t budget = std::min<size_t>(r->max_lds, (size_t)r->cache_kb << 10);
+        size_t avail = budget > base ? budget - base : 0;
+        int log2s = 0;
+        while (((size_t)2 << log2s) * slot_bytes <= avail && log2s < 30)
+            ++log2s;
+        if (((size_t)1 << log2s) * slot_bytes > avail)
+            want = false;
+        else {
+            p.log2s = log2s;
+            p.slots = 1 << log2s;
+            p.ways = 4;
+            if ((uint64_t)p.slots >= r->cells) {
+                while (p.log2s > 0 && ((uint64_t)1 << (p.log2s - 1)) >= r->cells)
+                    --p.log2s;
+                p.slots = 1 << p.log2s;
+                p.direct = 1;
+                p.ways = 1;
+            }
+        }
+    }
+    p.cache = want;
+    if (p.cache) {
+        p.lds = base + (size_t)p.slots * slot_bytes;
+        int per_cu = std::max<int>(1, (int)(r->lds_per_cu / std::max<size_t>(p.lds, 1)));
+        uint64_t lanes = (uint64_t)r->cu_count * (uint64_t)per_cu * (uint64_t)p.block;
+        uint64_t grid64 = std::min<uint64_t>((walkers + p.block - 1) / p.block,
+                                             lanes / (uint64_t)p.block);
+        p.grid = (int)std::max<uint64_t>(grid64, 1);
+        // walkers per launch: each lane takes at most `rounds` walkers, so a
+        // workgroup adds at most rounds * block * batch < 2^31 to one cell
+        uint64_t rounds = std::max<uint64_t>(1, ((1ull << 31) - 1) / ((uint64_t)p.block * batch));
+        p.walkers_per_launch = rounds * (uint64_t)p.grid * (uint64_t)p.block;
+    } else {
+        p.block = walkers >= (uint64_t)r->cu_count * 256 ? 256 : 64;
+        if (r->block_pref > 0 && r->block_pref <= 256)
+            p.block = r->block_pref;
+        uint64_t grid64 = (walkers + (uint64_t)p.block - 1) / (uint64_t)p.block;
+        p.grid = (int)std::min<uint64_t>(grid64, 1u << 30);
+        p.lds = r->lds_bytes;
+        p.walkers_per_launch = walkers;
+    }
+    return p;
+}
+
+template <typename R>
+static int launch_chunk(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
+                        uint64_t walker0, uint64_t n_walkers, uint64_t batch, uint64_t last)
+{
+    WalkArgs<R> a;
+    fill_args(r, d, a);
+    a.walker_base = walker0;
+    a.n_walkers = n_walkers;
+    a.batch = batch;
+    a.last_samples = last;
+    a.cache_slots = p.slots;
+    a.cache_log2 = p.log2s;
+    a.cache_ways = p.ways;
+    a.cache_direct = p.direct;
+    const int dims = r->spec.dims, nc = r->spec.color_dims;
+    int grid = (int)std::min<uint64_t>((uint64_t)p.grid, (n_walkers + p.block - 1) / p.block);
+    if constexpr (std::is_same_v<R, double>) {
+        return dims == 1 ? launch_walk_f64u64_d1(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
+             : dims == 2 ? launch_walk_f64u64_d2(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
+                         : launch_walk_f64u64_d3(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream);
+    } else if (r->mode == MODE_F32_U32) {
+        return dims == 1 ? launch_walk_f32u32_d1(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
+             : dims == 2 ? launch_walk_f32u32_d2(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
+                         : launch_walk_f32u32_d3(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream);
+    } else {
+        return dims == 1 ? launch_walk_f32u64_d1(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
+             : dims == 2 ? launch_walk_f32u64_d2(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
+                         : launch_walk_f32u64_d3(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream);
+    }
+}
+
 static int launch(ffr_renderer *r, uint64_t samples)
 {
     uint64_t batch = r->opt.batch_size ? r->opt.batch_size : ffr_auto_batch(r, samples);
     uint64_t walkers = (samples + batch - 1) / batch;
     uint64_t last = samples - (walkers - 1) * batch;
-    const int block = walkers >= (uint64_t)r->cu_count * 256 ? 256 : 64;
-    uint64_t grid64 = (walkers + (uint64_t)block - 1) / (uint64_t)block;
-    const int grid = (int)std::min<uint64_t>(grid64, 1u << 30);
+    LaunchPlan p = plan_launch(r, walkers, batch);
     r->last_walkers = walkers;
     r->last_batch = batch;
+    r->last_cache = p.cache ? p.slots : 0;
     int rc = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream);
     if (rc)
         return fail(FFR_E_DEVICE, std::string("stats reset: ") + hipGetErrorString((hipError_t)rc));
     HIP_TRY(hipEventRecord(r->ev0, r->stream));
-    const int dims = r->spec.dims, nc = r->spec.color_dims;
-    if (r->mode == MODE_F64_U64) {
-        WalkArgs<double> a;
-        fill_args(r, r->f64, a);
-        a.walker_base = r->next_walker;
-        a.n_walkers = walkers;
-        a.batch = batch;
-        a.last_samples = last;
-        rc = dims == 1 ? launch_walk_f64u64_d1(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
-           : dims == 2 ? launch_walk_f64u64_d2(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
-                       : launch_walk_f64u64_d3(nc, r->vs, a, grid, block, r->lds_bytes, r->stream);
-    } else {
-        WalkArgs<float> a;
-        fill_args(r, r->f32, a);
-        a.walker_base = r->next_walker;
-        a.n_walkers = walkers;
-        a.batch = batch;
-        a.last_samples = last;
-        if (r->mode == MODE_F32_U32)
-            rc = dims == 1 ? launch_walk_f32u32_d1(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
-               : dims == 2 ? launch_walk_f32u32_d2(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
-                           : launch_walk_f32u32_d3(nc, r->vs, a, grid, block, r->lds_bytes, r->stream);
-        else
-            rc = dims == 1 ? launch_walk_f32u64_d1(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
-               : dims == 2 ? launch_walk_f32u64_d2(nc, r->vs, a, grid, block, r->lds_bytes, r->stream)
-                           : launch_walk_f32u64_d3(nc, r->vs, a, grid, block, r->lds_bytes, r->stream);
+    for (uint64_t w0 = 0; w0 < walkers; w0 += p.walkers_per_launch) {
+        uint64_t n = std::min(p.walkers_per_launch, walkers - w0);
+        uint64_t chunk_last = (w0 + n == walkers) ? last : batch;
+        rc = r->mode == MODE_F64_U64
+                 ? launch_chunk(r, r->f64, p, r->next_walker + w0, n, batch, chunk_last)
+                 : launch_chunk(r, r->f32, p, r->next_walker + w0, n, batch, chunk_last);
+        if (rc)
+            return fail(FFR_E_DEVICE, std::string("walker kernel launch: ") +
+                                          hipGetErrorString((hipError_t)rc));
     }
-    if (rc)
-        return fail(FFR_E_DEVICE, std::string("walker kernel launch: ") +
-                                      hipGetErrorString((hipError_t)rc));
     HIP_TRY(hipEventRecord(r->ev1, r->stream));
     r->next_walker += walkers;
     r->pending = true;

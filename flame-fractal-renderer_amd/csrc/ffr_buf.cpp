@@ -40,7 +40,7 @@ struct Args {
     uint64_t seed = 0;
     std::string precision = "fp64";
     int word = 0; // 0: default for the precision
-    uint32_t settle = 0;
+    int32_t settle = -1;
     int device = 0;
     bool help = false;
     bool have_flame = false, have_output = false;
@@ -60,7 +60,7 @@ void usage(std::ostream &os)
           "  --seed arg (=0)               key of the walkers' random streams\n"
           "  --precision arg (=fp64)       trajectory precision: fp64 or fp32\n"
           "  --word arg                    buffer word bits: 64 (default) or 32\n"
-          "  --settle arg (=0)             settle iterations (0: 53 for fp64, 24 for fp32)\n"
+          "  --settle arg                  settle iterations (default 53 for fp64, 24 for fp32)\n"
           "  --device arg (=0)             HIP device\n";
 }
 
@@ -181,7 +181,7 @@ int parse_args(int argc, char **argv, Args &a)
             uint64_t s = 0;
             if (!need_u64(s))
                 return 1;
-            a.settle = (uint32_t)s;
+            a.settle = (int32_t)s;
         } else if (n == "device") {
             uint64_t d = 0;
             if (!need_u64(d))

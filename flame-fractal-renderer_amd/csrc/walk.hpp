@@ -41,6 +41,10 @@ template <typename R> struct WalkArgs {
     int nxf, n_slots, has_final, fin_nv, settle, r;
     int tab_reals;
     int xf0, fin0;
+    // LDS cell cache (kernels built with it): cache_slots entries of
+    // [key u32 | count u32 | r colour words], probed cache_ways deep; when
+    // cache_direct the cell index is the slot (the whole buffer fits)
+    int cache_slots, cache_log2, cache_ways, cache_direct;
     // slot and final-variation descriptors live in the kernel arguments so
     // the kernel reads them with scalar loads
     int slot_type[kMaxSlots];
@@ -52,6 +56,22 @@ template <typename R> struct WalkArgs {
     int l_preA, l_preb, l_postA, l_postb, l_speed, l_color, l_has_color, l_has_pre,
         l_has_post, l_stride;
 };
+
+constexpr unsigned kEmptyKey = 0xFFFFFFFFu;
+
+// byte offset of the LDS cell cache (after the flame table, the used-slot
+// masks and the per-xform counters), 8-byte aligned
+__host__ __device__ inline size_t cache_lds_offset(int tab_reals, int real_bytes, int nxf)
+{
+    size_t o = (size_t)tab_reals * (size_t)real_bytes + 8u * (size_t)nxf + 4u * (size_t)nxf;
+    return (o + 7u) & ~(size_t)7u;
+}
+
+// bytes of one cell-cache entry: key, count and the colour words
+__host__ __device__ inline size_t cache_slot_bytes(int r, int colour_bytes)
+{
+    return 8u + (size_t)r * (size_t)colour_bytes;
+}
 
 // precision modes (num_t / hist_t / colour word)
 enum { MODE_F64_U64 = 0, MODE_F32_U32 = 1, MODE_F32_U64 = 2 };

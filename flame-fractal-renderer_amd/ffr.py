@@ -35,7 +35,7 @@ class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_uint32),
         ("word", C.c_uint32),
-        ("settle_iters", C.c_uint32),
+        ("settle_iters", C.c_int32),
         ("device", C.c_int32),
         ("seed", C.c_uint64),
         ("batch_size", C.c_uint64),
@@ -154,7 +154,7 @@ class BufferRenderer:
 
     def __init__(self, flame_json: str, precision: str = "fp64", word: int = 64,
                  seed: int = 0, batch_size: int = 0, bad_value_limit: int = 256,
-                 settle: int = 0, device: int = 0, bad_value_cap: int = 4096):
+                 settle: int = -1, device: int = 0, bad_value_cap: int = 4096):
         L = lib()
         o = Options()
         L.ffr_default_options(C.byref(o))

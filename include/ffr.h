@@ -45,7 +45,7 @@ enum { FFR_WORD64 = 0, /* uint64 count + double colour sums (default build) */
 typedef struct ffr_options {
     uint32_t precision;      /* FFR_FP64 or FFR_FP32                                  */
     uint32_t word;           /* FFR_WORD64 or FFR_WORD32; FP64 needs FFR_WORD64       */
-    uint32_t settle_iters;   /* 0: reference default, 53 (FP64) / 24 (FP32)           */
+    int32_t settle_iters;    /* -1: reference default, 53 (FP64) / 24 (FP32)          */
     int32_t device;          /* HIP device ordinal                                    */
     uint64_t seed;           /* key of every walker's random stream                   */
     uint64_t batch_size;     /* samples per walker (ffr-buf -b); 0 = chosen for the GPU */
@@ -64,7 +64,7 @@ typedef struct ffr_stats {
     double kernel_ms;         /* device time of the last render's kernel (HIP events)  */
 } ffr_stats;
 
-/* defaults: FP64, WORD64, settle 0, device 0, seed 0, batch 0, limit 256 */
+/* defaults: FP64, WORD64, settle -1, device 0, seed 0, batch 0, limit 256 */
 void ffr_default_options(ffr_options *opt);
 
 /*

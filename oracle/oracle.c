@@ -565,6 +565,12 @@ int orc_render(orc_flame *f, void *buf, uint64_t seed, uint64_t walker_base,
              xf_dist, stats, bv_xfs, bv_pts, bv_cap)
 }
 
+int orc_apply_xform(orc_flame *f, int32_t xform, const double *in, double *out,
+                    uint64_t seed, uint64_t walker, uint64_t *draw_index)
+{
+    DISPATCH(applyxf, xform, in, out, seed, walker, draw_index)
+}
+
 int orc_eval_variation(orc_flame *f, int32_t xform, int32_t var,
                        const double *in, double *out, uint64_t seed,
                        uint64_t walker, uint64_t *draw_index)

@@ -103,7 +103,8 @@ typedef struct {
 
 /* build the optimised flame (drop zero weights, sort, cumulative weights) */
 orc_flame *orc_flame_new(const orc_flame_desc *desc, int32_t mode,
-                         int32_t settle_iters, char *err, size_t errlen);
+                         int32_t settle_iters /* -1: 53 / 24 */, char *err,
+                         size_t errlen);
 void orc_flame_free(orc_flame *f);
 
 /* introspection used by tests */
@@ -154,6 +155,10 @@ uint64_t orc_isaac64_next(orc_isaac64 *s);
 void orc_isaac32_seed_u32(orc_isaac32 *s, uint32_t seed);
 void orc_isaac32_seed_u64(orc_isaac32 *s, uint64_t seed);
 uint32_t orc_isaac32_next(orc_isaac32 *s);
+
+/* XForm::applyIteration on one point (xform = sorted index, -1 = final) */
+int orc_apply_xform(orc_flame *f, int32_t xform, const double *in, double *out,
+                    uint64_t seed, uint64_t walker, uint64_t *draw_index);
 
 /* variation evaluation on one point (tests of single variations) */
 int orc_eval_variation(orc_flame *f, int32_t xform, int32_t var,

@@ -141,6 +141,11 @@ def lib():
             C.POINTER(C.c_uint64)]
         L.orc_sort_order.argtypes = [C.POINTER(C.c_double), C.c_int32,
                                      C.c_int32, C.POINTER(C.c_int32)]
+        L.orc_apply_xform.restype = C.c_int
+        L.orc_apply_xform.argtypes = [
+            C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+            C.POINTER(C.c_double), C.c_uint64, C.c_uint64,
+            C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -209,7 +214,7 @@ class OracleFlame:
     """The oracle's optimised flame in one precision mode."""
 
     def __init__(self, raw: model.RawFlame | str, mode: int = F64_U64,
-                 settle: int = 0):
+                 settle: int = -1):
         if isinstance(raw, str):
             raw = model.parse_flame(raw, fp32=(mode != F64_U64))
         self.raw = raw
@@ -306,6 +311,18 @@ class OracleFlame:
             bad_xforms=np.array([bvx[i] for i in range(nb)], dtype=np.uint64),
             bad_points=np.array([[bvp[3 * i + k] for k in range(self.dims)]
                                  for i in range(nb)]).reshape(nb, self.dims))
+
+    def apply_xform(self, xform: int, point, seed: int = 0, walker: int = 0,
+                    draw: int = 0):
+        """XForm::applyIteration on one point (sorted xform index, -1 = final)."""
+        inp = (C.c_double * 3)(*([float(v) for v in point] + [0.0] * 3)[:3])
+        out = (C.c_double * 3)()
+        d = C.c_uint64(draw)
+        rc = lib().orc_apply_xform(self._h, xform, inp, out, seed, walker,
+                                   C.byref(d))
+        if rc != 0:
+            raise IndexError("no such xform")
+        return [out[i] for i in range(self.dims)], d.value
 
     def eval_variation(self, xform: int, var: int, point, seed: int = 0,
                        walker: int = 0, draw: int = 0):

@@ -1425,7 +1425,7 @@ static void *NM(flame_new)(const orc_flame_desc *d, int32_t settle, char *err,
     NM(flame) *f = (NM(flame) *)calloc(1, sizeof *f);
     f->dims = d->dims;
     f->r = d->color_dims;
-    f->settle = settle > 0 ? settle : SETTLE_DEFAULT;
+    f->settle = settle >= 0 ? settle : SETTLE_DEFAULT;
     R default_speed = (R)d->color_speed;
     int n = d->n_xforms;
     NM(xf) *all = (NM(xf) *)calloc((size_t)(n > 0 ? n : 1), sizeof(NM(xf)));
@@ -1820,6 +1820,26 @@ static int NM(render)(void *vf, void *buf, uint64_t seed, uint64_t walker_base,
     pthread_mutex_destroy(&P.mu);
     pthread_mutex_destroy(&sh.mu);
     return P.success ? 0 : 1;
+}
+
+static int NM(applyxf)(void *vf, int32_t xform, const double *in, double *out,
+                       uint64_t seed, uint64_t walker, uint64_t *draw_index)
+{
+    NM(flame) *f = (NM(flame) *)vf;
+    if (xform < -1 || xform >= f->nxf || (xform == -1 && !f->has_final))
+        return -1;
+    const NM(xf) *xf = (xform < 0) ? &f->fin : &f->xf[xform];
+    orc_stream st;
+    stream_init(&st, seed, walker);
+    st.n = *draw_index;
+    R p[3];
+    for (int i = 0; i < f->dims; ++i)
+        p[i] = (R)in[i];
+    NM(xf_apply)(xf, f->dims, p, &st);
+    for (int i = 0; i < f->dims; ++i)
+        out[i] = (double)p[i];
+    *draw_index = st.n;
+    return 0;
 }
 
 static int NM(evalvar)(void *vf, int32_t xform, int32_t var, const double *in,
