@@ -18,6 +18,12 @@ if PKG not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+    # build only when nothing is built (the GPU box runs what was built here)
+    need = [os.path.join(PKG, "libffr_hip.so"), os.path.join(PKG, "ffr-buf"),
+            os.path.join(ROOT, "oracle", "liboracle.so")]
+    if not all(os.path.exists(p) for p in need):
+        import __graft_entry__
+        __graft_entry__.build(force=False)
 
 
 def read_flame(name: str) -> str:
