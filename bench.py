@@ -78,6 +78,21 @@ def run_gpu(ffr, text, precision, steps, warmup):
     return out
 
 
+def calibration_pass(ffr, text):
+    """One buffer add of known size: the kernel reads 2 x and writes 1 x the
+    buffer (8294400 cells x 32 B); its FETCH_SIZE / WRITE_SIZE counters scale
+    the walker kernel's into bytes."""
+    import numpy as np
+
+    r = ffr.BufferRenderer(text, precision="fp32", seed=SEED)
+    zeros = np.zeros(r.cells * r.cell_words, dtype=np.uint64)
+    r.add_buffer(zeros)
+    r.histogram_stats()  # hist_stats_kernel reads every line of the buffer once
+    log(f"calibration: clear writes {r.buffer_bytes} bytes; histogram pass reads "
+        f"{r.buffer_bytes} bytes; add reads {2 * r.buffer_bytes}, writes {r.buffer_bytes}")
+    r.close()
+
+
 def cpu_baseline(text, seconds):
     from oracle import oracle
 
@@ -108,6 +123,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one buffer add of known size (for counter scaling)")
+    ap.add_argument("--fp32-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -119,8 +137,12 @@ def main():
     with open(FLAME) as fh:
         text = fh.read()
     log(f"bench: {SAMPLES} samples/step, warmup {args.warmup}, steps {args.steps}")
+    if args.calibrate:
+        calibration_pass(ffr, text)
     fp32 = run_gpu(ffr, text, "fp32", args.steps, args.warmup)
     log("fp32", json.dumps(fp32))
+    if args.fp32_only:
+        return 0
     fp64 = run_gpu(ffr, text, "fp64", args.steps, args.warmup)
     log("fp64", json.dumps(fp64))
     cpu = None if args.no_cpu else cpu_baseline(text, args.cpu_seconds)
