@@ -1,20 +1,34 @@
 """Benchmark: buffer-render throughput on one MI355X.
 
-Workload (one "step"): one render pass of 2^32 samples of the random 2-D
-colour flame (seed 2, 6 xforms x 3 variations from linear / sinusoidal /
-spherical / swirl / horseshoe / polar) into its 3840x2160 buffer of
-uint64 counts + 3 double colour sums per cell (the reference's default
-buffer layout), walkers chosen by the renderer.  Measured with FP32
-trajectories (the reported `value`) and with FP64 trajectories.
+Headline workload (one "step"): one render pass of 2^32 samples of the
+random 2-D colour flame (seed 2, 6 xforms x 3 variations from linear /
+sinusoidal / spherical / swirl / horseshoe / polar) into its 3840x2160
+buffer of uint64 counts + 3 double colour sums per cell (the reference's
+default buffer layout, what ffr-buf writes), walkers chosen by the
+renderer.  Measured with FP32 trajectories (the reported `value`) and with
+FP64 trajectories; `l1_fp32_vs_fp64` is the L1 distance between the two
+normalised histograms.
+
+Also measured (sub-objects of `workloads`):
+  - the Sierpinski triangle, 1024x1024 uint64 counts, 2^26 samples from
+    1024 walkers of 65536 with 20 settle iterations, FP64 and FP32;
+  - the 3-D Sierpinski tetrahedron, 512^3 uint32 counts, 2^33 samples,
+    FP32 trajectories.
 
 Prints one JSON line on stdout.  `roofline` is for the walker kernel:
-algorithmic bytes = plotted samples x 32 B (the 8-byte count and three
-8-byte colour words each sample adds into its cell) over the kernel's
-average duration from HIP events on its stream.  `cpu_baseline` times the
-CPU oracle (the reference loop restated in C, threads = host cores) on a
-bounded sample of the same workload.
+algorithmic bytes = plotted samples x the bytes each plot adds into its
+cell (32 B: an 8-byte count and three 8-byte colour sums; 8 B for the
+Sierpinski counts, 4 B for the tetrahedron counts) over the kernel's
+average duration from HIP events recorded on the renderer's stream.
+`traffic` is the walker kernel's HBM bytes per launch from the profiler's
+FETCH_SIZE and WRITE_SIZE counters on this workload (the profile summary
+under profiles/ names the command), scaled by the same counters on
+kernels of known traffic.  `cpu_baseline` times the CPU oracle (the
+reference loop restated in C, threads = host cores) on a bounded sample
+of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--only c2|c1|c3] [--fp32-only] [--calibrate]
 """
 
 from __future__ import annotations
@@ -30,36 +44,56 @@ PKG = os.path.join(ROOT, "flame-fractal-renderer_amd")
 sys.path.insert(0, PKG)
 sys.path.insert(0, ROOT)
 
-FLAME = os.path.join(PKG, "flames", "random_color_3840x2160.json")
-SAMPLES = 1 << 32
-SEED = 0x5EED0002
-BYTES_PER_PLOT = 32   # u64 count + 3 x f64 colour, added into one cell
+FLAMES = os.path.join(PKG, "flames")
+C2_FLAME = os.path.join(FLAMES, "random_color_3840x2160.json")
+C1_FLAME = os.path.join(FLAMES, "sierpinski_1024.json")
+C3_FLAME = os.path.join(FLAMES, "tetrahedron_512.json")
+C2_SAMPLES = 1 << 32
+C1_SAMPLES = 1 << 26
+C3_SAMPLES = 1 << 33
+C2_SEED, C1_SEED, C3_SEED = 0x5EED0002, 0x5EED0001, 0x5EED0003
 HBM_PEAK_GBS = 8000.0
+# walker kernel HBM bytes per launch on the headline workload (FP32):
+# FETCH_SIZE 521.5 KiB x 2.0 + WRITE_SIZE 18455193.9 KiB x 1.0 per dispatch.
+# The factors come from kernels of known traffic on the same buffer (the add
+# reads 2 x 265420800 B and reports 259236 KiB of FETCH_SIZE, the histogram
+# pass reads the buffer once and reports half of it; the clear and the add
+# each write 265420800 B and report exactly 259200 KiB of WRITE_SIZE).
+# Global atomics count 32 B each in WRITE_SIZE.  The raw values are in
+# profiles/walk_kernel_random_color_3840x2160_hbm.md.
+C2_FP32_TRAFFIC_BYTES = 521.5 * 1024 * 2.0 + 18455193.9375 * 1024 * 1.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(ffr, text, precision, steps, warmup):
-    r = ffr.BufferRenderer(text, precision=precision, seed=SEED)
+def read(path):
+    with open(path) as fh:
+        return fh.read()
+
+
+def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, seed,
+            word=64, batch=0, settle=-1, keep=False):
+    r = ffr.BufferRenderer(text, precision=precision, word=word, seed=seed,
+                           batch_size=batch, settle=settle)
     for _ in range(warmup):
-        if not r.render(SAMPLES):
+        if not r.render(samples):
             raise RuntimeError("render stopped at the bad-value limit")
     plot0 = r.samples_plotted
     kms = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        if not r.render(SAMPLES):
+        if not r.render(samples):
             raise RuntimeError("render stopped at the bad-value limit")
         kms.append(r.kernel_ms)
     t1 = time.perf_counter()
     plotted = (r.samples_plotted - plot0) / steps
     wall = t1 - t0
     kavg = sum(kms) / len(kms)
-    achieved = plotted * BYTES_PER_PLOT / (kavg * 1e-3) / 1e9
+    achieved = plotted * bytes_per_plot / (kavg * 1e-3) / 1e9
     out = {
-        "value": steps * SAMPLES / wall,
+        "value": steps * samples / wall,
         "ms_per_step": 1e3 * wall / steps,
         "kernel_ms": kavg,
         "walkers": int(r.last_stats.walkers),
@@ -74,46 +108,54 @@ def run_gpu(ffr, text, precision, steps, warmup):
             "traffic": None,
         },
     }
+    hist = None
+    if keep:
+        hist = r.counts().astype("float64")
+        hist /= max(hist.sum(), 1.0)
     r.close()
-    return out
+    return out, hist
+
+
+def l1(a, b):
+    import numpy as np
+    return float(np.abs(a - b).sum())
 
 
 def calibration_pass(ffr, text):
-    """One buffer add of known size: the kernel reads 2 x and writes 1 x the
-    buffer (8294400 cells x 32 B); its FETCH_SIZE / WRITE_SIZE counters scale
-    the walker kernel's into bytes."""
+    """Kernels of known traffic on the headline buffer (8294400 cells x 32 B):
+    the clear writes it once, the histogram statistics read it once, the add
+    reads it twice and writes it once.  Their FETCH_SIZE / WRITE_SIZE
+    counters scale the walker kernel's into bytes."""
     import numpy as np
 
-    r = ffr.BufferRenderer(text, precision="fp32", seed=SEED)
+    r = ffr.BufferRenderer(text, precision="fp32", seed=C2_SEED)
     zeros = np.zeros(r.cells * r.cell_words, dtype=np.uint64)
     r.add_buffer(zeros)
-    r.histogram_stats()  # hist_stats_kernel reads every line of the buffer once
+    r.histogram_stats()
     log(f"calibration: clear writes {r.buffer_bytes} bytes; histogram pass reads "
         f"{r.buffer_bytes} bytes; add reads {2 * r.buffer_bytes}, writes {r.buffer_bytes}")
     r.close()
 
 
-def cpu_baseline(text, seconds):
+def cpu_rate(text, mode, seconds, seed, batch, settle=-1, fixed=None):
+    """Samples/s of the CPU oracle on all host threads: either `fixed`
+    samples, or about `seconds` of work sized from a short probe."""
     from oracle import oracle
 
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    f = oracle.OracleFlame(text, oracle.F64_U64)
-    n = 1 << 20
+    f = oracle.OracleFlame(text, mode, settle=settle)
+    if fixed is None:
+        n = 1 << 24
+        t0 = time.perf_counter()
+        f.render(n, batch, seed=seed, threads=cores)
+        rate = n / (time.perf_counter() - t0)
+        n = int(min(max(rate * seconds, 1 << 20), 1 << 31)) // batch * batch
+    else:
+        n = fixed
     t0 = time.perf_counter()
-    f.render(n, 8192, seed=SEED, threads=cores)
-    rate = n / (time.perf_counter() - t0)
-    n = int(min(max(rate * seconds, 1 << 20), 1 << 31)) // 8192 * 8192
-    t0 = time.perf_counter()
-    f.render(n, 8192, seed=SEED, threads=cores)
+    f.render(n, batch, seed=seed, threads=cores)
     dt = time.perf_counter() - t0
-    return {
-        "value": n / dt,
-        "unit": "samples/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"{n} samples of the same flame in FP64 (walkers of 8192), "
-                  f"{dt:.1f} s on {cores} host threads",
-    }
+    return n / dt, n, dt, cores
 
 
 def main():
@@ -123,38 +165,91 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--only", choices=["c1", "c2", "c3"], default=None,
+                    help="run one workload only (profiling)")
     ap.add_argument("--calibrate", action="store_true",
-                    help="also run one buffer add of known size (for counter scaling)")
+                    help="also run kernels of known traffic (counter scaling)")
     ap.add_argument("--fp32-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(walker ranges would shard across GPUs with one integer sum of the "
-              "buffers at the end; see DESIGN.md)", file=sys.stderr)
+              "buffers at the end)", file=sys.stderr)
         return 2
     import ffr
+    from oracle import oracle
 
-    with open(FLAME) as fh:
-        text = fh.read()
-    log(f"bench: {SAMPLES} samples/step, warmup {args.warmup}, steps {args.steps}")
+    S, W = args.steps, args.warmup
+    want = (lambda k: args.only in (None, k))
+    c2_text = read(C2_FLAME)
     if args.calibrate:
-        calibration_pass(ffr, text)
-    fp32 = run_gpu(ffr, text, "fp32", args.steps, args.warmup)
-    log("fp32", json.dumps(fp32))
-    if args.fp32_only:
+        calibration_pass(ffr, c2_text)
+
+    fp32 = fp64 = None
+    l1_c2 = None
+    if want("c2"):
+        log(f"bench c2: {C2_SAMPLES} samples/step, warmup {W}, steps {S}")
+        fp32, h32 = run_gpu(ffr, c2_text, precision="fp32", samples=C2_SAMPLES, steps=S,
+                            warmup=W, bytes_per_plot=32, seed=C2_SEED,
+                            keep=not args.fp32_only)
+        fp32["roofline"]["traffic"] = C2_FP32_TRAFFIC_BYTES
+        log("c2 fp32", json.dumps(fp32))
+        if not args.fp32_only:
+            fp64, h64 = run_gpu(ffr, c2_text, precision="fp64", samples=C2_SAMPLES, steps=S,
+                                warmup=W, bytes_per_plot=32, seed=C2_SEED, keep=True)
+            l1_c2 = l1(h32, h64)
+            del h32, h64
+            log("c2 fp64", json.dumps(fp64), "L1", l1_c2)
+
+    workloads = {}
+    if want("c1") and not args.fp32_only:
+        text = read(C1_FLAME)
+        for prec in ("fp64", "fp32"):
+            log(f"bench c1 {prec}: {C1_SAMPLES} samples/step")
+            res, h = run_gpu(ffr, text, precision=prec, samples=C1_SAMPLES, steps=S, warmup=W,
+                             bytes_per_plot=8, seed=C1_SEED, batch=65536, settle=20, keep=True)
+            res["hist"] = h
+            workloads[f"sierpinski_1024_{prec}"] = res
+        h64 = workloads["sierpinski_1024_fp64"].pop("hist")
+        h32 = workloads["sierpinski_1024_fp32"].pop("hist")
+        workloads["sierpinski_1024_fp32"]["l1_vs_fp64"] = l1(h32, h64)
+        if not args.no_cpu:
+            rate, n, dt, cores = cpu_rate(text, oracle.F64_U64, 0, C1_SEED, 65536, settle=20,
+                                          fixed=C1_SAMPLES)
+            workloads["sierpinski_1024_fp64"]["cpu"] = {
+                "value": rate, "cores": cores, "kind": "port",
+                "sample": f"the full {n} samples in FP64, {dt:.2f} s"}
+        for k in ("sierpinski_1024_fp64", "sierpinski_1024_fp32"):
+            log(k, json.dumps(workloads[k]))
+    if want("c3") and not args.fp32_only:
+        text = read(C3_FLAME)
+        log(f"bench c3: {C3_SAMPLES} samples/step")
+        res, _ = run_gpu(ffr, text, precision="fp32", word=32, samples=C3_SAMPLES, steps=S,
+                         warmup=W, bytes_per_plot=4, seed=C3_SEED)
+        workloads["tetrahedron_512_fp32"] = res
+        log("tetrahedron_512_fp32", json.dumps(res))
+
+    if fp32 is None:
         return 0
-    fp64 = run_gpu(ffr, text, "fp64", args.steps, args.warmup)
-    log("fp64", json.dumps(fp64))
-    cpu = None if args.no_cpu else cpu_baseline(text, args.cpu_seconds)
-    if cpu:
+    cpu = None
+    if not args.no_cpu and not args.fp32_only:
+        rate, n, dt, cores = cpu_rate(c2_text, oracle.F64_U64, args.cpu_seconds, C2_SEED, 8192)
+        cpu = {
+            "value": rate,
+            "unit": "samples/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"{n} samples of the same flame in FP64 (walkers of 8192), "
+                      f"{dt:.1f} s on {cores} host threads",
+        }
         log("cpu", json.dumps(cpu))
     line = {
         "metric": "samples/sec and fraction of atomic/HBM roofline at FP32 vs FP64, 1 GPU; p99 latency",
         "value": fp32["value"],
         "unit": "samples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": S,
+        "warmup": W,
         "ms_per_step": fp32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
@@ -170,12 +265,14 @@ def main():
         },
         "roofline": fp32["roofline"],
         "kernel_ms": fp32["kernel_ms"],
-        "fp64": {
+        "fp64": None if fp64 is None else {
             "value": fp64["value"],
             "ms_per_step": fp64["ms_per_step"],
             "kernel_ms": fp64["kernel_ms"],
             "roofline": fp64["roofline"],
         },
+        "l1_fp32_vs_fp64": l1_c2,
+        "workloads": workloads,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -124,8 +124,7 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
     HIP_TRY(hipMalloc(&r->tab.masks, sizeof(unsigned long long) * (size_t)d.nxf));
     HIP_TRY(hipMemcpy(r->tab.masks, d.masks.data(), sizeof(unsigned long long) * (size_t)d.nxf,
                       hipMemcpyHostToDevice));
-    r->lds_bytes = r->tab.table_bytes + sizeof(unsigned long long) * (size_t)d.nxf +
-                   sizeof(unsigned int) * (size_t)d.nxf;
+    r->lds_bytes = cache_lds_offset(r->tab.tab_reals, (int)sizeof(R), d.nxf);
     int max_lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                   r->device));
@@ -151,6 +150,7 @@ void ffr_default_options(ffr_options *opt)
     opt->settle_iters = -1;
     opt->bad_value_limit = 256;
     opt->bad_value_cap = 4096;
+    opt->cell_cache = -1;
 }
 
 const char *ffr_last_error(void) { return g_err.c_str(); }
@@ -202,6 +202,11 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
             r->lds_per_cu = (size_t)v;
         else
             r->lds_per_cu = r->max_lds;
+        r->cache_pref = opt.cell_cache < 0 ? -1 : (opt.cell_cache ? 1 : 0);
+        if (opt.cache_kib > 0)
+            r->cache_kb = opt.cache_kib;
+        if (opt.block > 0)
+            r->block_pref = opt.block;
         if (const char *e = std::getenv("FFR_CACHE"))
             r->cache_pref = std::atoi(e);
         if (const char *e = std::getenv("FFR_CACHE_KB"))
@@ -339,6 +344,10 @@ static void fill_args(ffr_renderer *r, const DeviceFlame<R> &d, WalkArgs<R> &a)
         a.slot_type[s] = d.slot_type[s];
         a.slot_off[s] = d.slot_off[s];
         a.slot_stride[s] = d.slot_stride[s];
+        const int code = basic_code(d.slot_type[s]);
+        if (d.slot_type[s] >= 0 && code >= 0 && code < kBasicTypes &&
+            basic_type(code) == d.slot_type[s])
+            a.basic_slots[code] |= 1ull << s;
     }
     for (size_t j = 0; j < d.fin_type.size(); ++j) {
         a.fin_type[j] = d.fin_type[j];
@@ -373,7 +382,12 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
     const int colour_bytes = r->word_bytes;
     const size_t slot_bytes = cache_slot_bytes(r->spec.color_dims, colour_bytes);
     bool want = r->cache_pref != 0 && r->cells < 0xFFFFFFFFull && r->spec.color_dims <= 4;
-    p.block = r->block_pref > 0 ? r->block_pref : 512;
+    // with fewer walkers than 256 per CU the render is bound by the latency
+    // of one walker's chain, which the cache lengthens (unless forced on)
+    if (r->cache_pref < 0 && walkers < (uint64_t)r->cu_count * 256u)
+        want = false;
+    const int bmax = cache_block_max(r->mode == MODE_F64_U64, r->vs);
+    p.block = (r->block_pref > 0 && r->block_pref <= bmax) ? r->block_pref : bmax;
     if (want && (uint64_t)p.block * batch >= (1ull << 31))
         want = false;
     if (want) {

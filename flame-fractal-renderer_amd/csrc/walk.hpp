@@ -52,6 +52,9 @@ template <typename R> struct WalkArgs {
     int slot_stride[kMaxSlots]; // entry size
     int fin_type[kMaxFinalVars];
     int fin_off[kMaxFinalVars];
+    // kernels for the six basic variations in 2-D: the slots of each basic
+    // type (bit s set when slot s has that type), by basic_code
+    unsigned long long basic_slots[6];
     // xform record layout (flame_model.hpp XformLayout)
     int l_preA, l_preb, l_postA, l_postb, l_speed, l_color, l_has_color, l_has_pre,
         l_has_post, l_stride;
@@ -59,12 +62,31 @@ template <typename R> struct WalkArgs {
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;
 
-// byte offset of the LDS cell cache (after the flame table, the used-slot
-// masks and the per-xform counters), 8-byte aligned
+// the basic variation set: linear, sinusoidal, spherical, swirl, horseshoe,
+// polar (type ids 0, 1, 2, 20, 21, 22) and their codes 0..5
+constexpr int kBasicTypes = 6;
+__host__ __device__ constexpr int basic_type(int code) { return code < 3 ? code : code + 17; }
+__host__ __device__ constexpr int basic_code(int type) { return type < 3 ? type : type - 17; }
+
+// largest workgroup of the kernels that accumulate through the LDS cell
+// cache: 1024 lanes (16 waves share one cache) where the kernel fits 128
+// VGPRs, 512 for the FP64 and all-variation kernels, which need more
+__host__ __device__ constexpr int cache_block_max(bool fp64, int vs)
+{
+    return (fp64 || vs == 2) ? 512 : 1024;
+}
+
+// LDS layout of the walker kernel: the flame table, then (8-byte aligned)
+// the used-slot mask of each xform (u64), the per-xform selection counters
+// (u64), then the cell cache
+__host__ __device__ inline size_t lds_masks_offset(int tab_reals, int real_bytes)
+{
+    return ((size_t)tab_reals * (size_t)real_bytes + 7u) & ~(size_t)7u;
+}
+
 __host__ __device__ inline size_t cache_lds_offset(int tab_reals, int real_bytes, int nxf)
 {
-    size_t o = (size_t)tab_reals * (size_t)real_bytes + 8u * (size_t)nxf + 4u * (size_t)nxf;
-    return (o + 7u) & ~(size_t)7u;
+    return lds_masks_offset(tab_reals, real_bytes) + 16u * (size_t)nxf;
 }
 
 // bytes of one cell-cache entry: key, count and the colour words

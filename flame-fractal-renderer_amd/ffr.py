@@ -41,6 +41,10 @@ class Options(C.Structure):
         ("batch_size", C.c_uint64),
         ("bad_value_limit", C.c_uint64),
         ("bad_value_cap", C.c_uint64),
+        ("cell_cache", C.c_int32),
+        ("cache_kib", C.c_int32),
+        ("block", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -154,7 +158,8 @@ class BufferRenderer:
 
     def __init__(self, flame_json: str, precision: str = "fp64", word: int = 64,
                  seed: int = 0, batch_size: int = 0, bad_value_limit: int = 256,
-                 settle: int = -1, device: int = 0, bad_value_cap: int = 4096):
+                 settle: int = -1, device: int = 0, bad_value_cap: int = 4096,
+                 cell_cache: int = -1, cache_kib: int = 0, block: int = 0):
         L = lib()
         o = Options()
         L.ffr_default_options(C.byref(o))
@@ -166,6 +171,9 @@ class BufferRenderer:
         o.batch_size = batch_size
         o.bad_value_limit = bad_value_limit
         o.bad_value_cap = bad_value_cap
+        o.cell_cache = cell_cache
+        o.cache_kib = cache_kib
+        o.block = block
         h = C.c_void_p()
         b = flame_json.encode()
         _check(L.ffr_create(b, len(b), C.byref(o), C.byref(h)))

@@ -51,6 +51,14 @@ typedef struct ffr_options {
     uint64_t batch_size;     /* samples per walker (ffr-buf -b); 0 = chosen for the GPU */
     uint64_t bad_value_limit;/* ffr-buf -z (default 256)                              */
     uint64_t bad_value_cap;  /* bad-value records kept per render (default 4096)      */
+    /* launch shape (no counterpart in the reference; results do not depend
+     * on it): accumulation through the per-workgroup LDS cell cache
+     * (-1 chosen per render, 0 off, 1 on where the flame allows it), the
+     * LDS the cache may take in KiB (0: 144), lanes per workgroup (0: chosen) */
+    int32_t cell_cache;
+    int32_t cache_kib;
+    int32_t block;
+    int32_t reserved;
 } ffr_options;
 
 typedef struct ffr_stats {
@@ -64,7 +72,8 @@ typedef struct ffr_stats {
     double kernel_ms;         /* device time of the last render's kernel (HIP events)  */
 } ffr_stats;
 
-/* defaults: FP64, WORD64, settle -1, device 0, seed 0, batch 0, limit 256 */
+/* defaults: FP64, WORD64, settle -1, device 0, seed 0, batch 0, limit 256,
+ * cell_cache -1, cache_kib 0, block 0 */
 void ffr_default_options(ffr_options *opt);
 
 /*

@@ -65,12 +65,18 @@ def l1(a, b):
 
 # ------------------------------------------------------------ bit-exact, linear
 
+# cell_cache: -1 lets the renderer choose (off for these few walkers), 1
+# forces accumulation through the LDS cell cache
+CACHE = [-1, 1]
+
+
+@pytest.mark.parametrize("cache", CACHE)
 @pytest.mark.parametrize("mode", ["fp64", "fp32", "fp32u64"])
-def test_sierpinski_bit_exact(ffr_mod, oracle_mod, mode):
+def test_sierpinski_bit_exact(ffr_mod, oracle_mod, mode, cache):
     text = sierpinski(256)
     samples, batch = 1 << 20, 1 << 14
-    print(f"sierpinski 256^2 {mode}", flush=True)
-    r, ok = gpu_render(ffr_mod, text, samples, batch, mode)
+    print(f"sierpinski 256^2 {mode} cache {cache}", flush=True)
+    r, ok = gpu_render(ffr_mod, text, samples, batch, mode, cell_cache=cache)
     ref = oracle_render(oracle_mod, text, samples, batch, mode)
     assert ok and ref.success
     assert np.array_equal(r.read_buffer(), ref.buffer)
@@ -107,11 +113,12 @@ def test_config1_full_size_bit_exact(ffr_mod, oracle_mod):
     ("tied_weights", TF.tied_weights, 2),
 ])
 @pytest.mark.parametrize("mode", ["fp64", "fp32"])
-def test_linear_flames_bit_exact(ffr_mod, oracle_mod, name, maker, dims, mode):
+@pytest.mark.parametrize("cache", CACHE)
+def test_linear_flames_bit_exact(ffr_mod, oracle_mod, name, maker, dims, mode, cache):
     text = maker()
     samples, batch = 300_000, 3000
-    print(f"{name} {mode}", flush=True)
-    r, ok = gpu_render(ffr_mod, text, samples, batch, mode, settle=-1)
+    print(f"{name} {mode} cache {cache}", flush=True)
+    r, ok = gpu_render(ffr_mod, text, samples, batch, mode, settle=-1, cell_cache=cache)
     ref = oracle_render(oracle_mod, text, samples, batch, mode, settle=-1)
     assert ok == ref.success
     assert np.array_equal(r.read_buffer(), ref.buffer)
@@ -121,11 +128,12 @@ def test_linear_flames_bit_exact(ffr_mod, oracle_mod, name, maker, dims, mode):
 
 @pytest.mark.parametrize("r_dims,final_color", [(3, True), (3, False), (1, True), (5, True)])
 @pytest.mark.parametrize("mode", ["fp64", "fp32u64", "fp32"])
-def test_color_flames(ffr_mod, oracle_mod, r_dims, final_color, mode):
+@pytest.mark.parametrize("cache", CACHE)
+def test_color_flames(ffr_mod, oracle_mod, r_dims, final_color, mode, cache):
     text = TF.color_flame(r_dims, final_color)
     samples, batch = 400_000, 2000
-    print(f"colour r={r_dims} final_color={final_color} {mode}", flush=True)
-    r, ok = gpu_render(ffr_mod, text, samples, batch, mode, settle=-1)
+    print(f"colour r={r_dims} final_color={final_color} {mode} cache {cache}", flush=True)
+    r, ok = gpu_render(ffr_mod, text, samples, batch, mode, settle=-1, cell_cache=cache)
     ref = oracle_render(oracle_mod, text, samples, batch, mode, settle=-1, threads=1)
     assert ok and ref.success
     words = r.read_buffer()
@@ -242,6 +250,35 @@ def test_transcendental_flame_l1(ffr_mod, oracle_mod):
     # streams keep the trajectories close; independent ones do not)
     assert d64 <= 1e-3
     assert d32 <= 0.1 * floor
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("mode", ["fp64", "fp32u64"])
+def test_variation_sum_order(ffr_mod, oracle_mod, mode, reverse):
+    """Several basic variations in one xform: the weighted sum runs in the
+    JSON order (xform.hpp:214-221) on the device too.  A constant pre-affine
+    maps every point to t = (1, 1); linear gives t, spherical t / (2 + eps)
+    = (0.5, 0.5).  With e = 2^-24 (float) or 2^-53 (double):
+      ((0 + 1) + 0.5 e) + e = 1          (ties to even, twice)
+      ((0 + e) + 0.5 e) + 1 = 1 + 2 e    (1.5 e rounds up to the next float)
+    """
+    e = 2.0 ** -24 if mode != "fp64" else 2.0 ** -53
+    vs = [{"name": "linear", "weight": 1.0}, {"name": "spherical", "weight": e},
+          {"name": "linear", "weight": e}]
+    if reverse:
+        vs = vs[::-1]
+    text = json.dumps({"dimensions": 2, "size": [8, 8], "bounds": [[-4, 4], [-4, 4]],
+                       "xforms": [{"weight": 1, "variations": vs,
+                                   "pre_affine": {"A": [[0, 0], [0, 0]], "b": [1, 1]}}]})
+    want = 1.0 + 2 * e if reverse else 1.0
+    r = ffr_mod.BufferRenderer(text, precision="fp64" if mode == "fp64" else "fp32",
+                               seed=SEED, batch_size=1, settle=0)
+    assert r.render(1)
+    ref = oracle_mod.OracleFlame(text, MODES[mode], settle=0).render(1, 1, seed=SEED)
+    for i in range(2):
+        assert r.point_extremes[i] == (want, want), r.point_extremes
+        assert ref.pt_min[i] == want and ref.pt_max[i] == want
+    r.close()
 
 
 # ------------------------------------------------- every variation
