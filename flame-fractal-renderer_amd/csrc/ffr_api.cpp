@@ -81,6 +81,7 @@ struct ffr_renderer {
     std::vector<uint64_t> bv_ids;
     std::vector<double> bv_pts;
     bool pending = false;
+    bool timed = false; // ev0/ev1 bracket the pending render (a batch times itself)
     int nxf = 0;
     std::vector<int> order; // sorted position -> xform id
     // launch policy: LDS cell cache (-1 auto, 0 off, 1 on), its LDS budget in
@@ -355,6 +356,23 @@ static void fill_args(ffr_renderer *r, const DeviceFlame<R> &d, WalkArgs<R> &a)
     }
 }
 
+// launch shape: what the launch may use; results do not depend on it
+struct LaunchShape {
+    int cache = -1;     // LDS cell cache: -1 chosen, 0 off, 1 on
+    int cache_kb = 144; // LDS budget of the cache per workgroup, KiB
+    int block = 0;      // lanes per workgroup, 0 chosen
+    int task_rounds = 1; // walkers per lane in one workgroup task
+};
+
+static LaunchShape renderer_shape(const ffr_renderer *r)
+{
+    LaunchShape sh;
+    sh.cache = r->cache_pref;
+    sh.cache_kb = r->cache_kb;
+    sh.block = r->block_pref;
+    return sh;
+}
+
 // launch geometry of one render pass (see plan_launch)
 struct LaunchPlan {
     bool cache = false;
@@ -362,7 +380,7 @@ struct LaunchPlan {
     int grid = 1;
     size_t lds = 0;
     int slots = 0, log2s = 0, ways = 1, direct = 0;
-    uint64_t walkers_per_launch = 0;
+    uint64_t task_walkers = 256;
 };
 
 static size_t table_lds(const ffr_renderer *r)
@@ -372,26 +390,31 @@ static size_t table_lds(const ffr_renderer *r)
 }
 
 // Accumulation through the LDS cell cache needs cell indices below 2^32,
-// at most 4 colour words, and per-workgroup counts that fit 32 bits.  The
+// at most 4 colour words, and per-task counts that fit 32 bits (a task of
+// task_walkers walkers adds at most task_walkers * batch to one cell).  The
 // cache takes the LDS left after the flame table, up to the budget; a
-// buffer that fits entirely is indexed directly.
-static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t batch)
+// buffer that fits entirely is indexed directly.  `base_lds` is the LDS the
+// launch must hold besides the cache (the largest flame table of a batch).
+static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t batch,
+                              const LaunchShape &sh, size_t base_lds = 0)
 {
     LaunchPlan p;
-    const size_t base = table_lds(r);
+    const size_t base = std::max(table_lds(r), base_lds);
     const int colour_bytes = r->word_bytes;
     const size_t slot_bytes = cache_slot_bytes(r->spec.color_dims, colour_bytes);
-    bool want = r->cache_pref != 0 && r->cells < 0xFFFFFFFFull && r->spec.color_dims <= 4;
+    bool want = sh.cache != 0 && r->cells < 0xFFFFFFFFull && r->spec.color_dims <= 4;
     // with fewer walkers than 256 per CU the render is bound by the latency
     // of one walker's chain, which the cache lengthens (unless forced on)
-    if (r->cache_pref < 0 && walkers < (uint64_t)r->cu_count * 256u)
+    if (sh.cache < 0 && walkers < (uint64_t)r->cu_count * 256u)
         want = false;
     const int bmax = cache_block_max(r->mode == MODE_F64_U64, r->vs);
-    p.block = (r->block_pref > 0 && r->block_pref <= bmax) ? r->block_pref : bmax;
+    p.block = (sh.block > 0 && sh.block <= bmax) ? sh.block : bmax;
+    const uint64_t rounds = (uint64_t)std::max(1, sh.task_rounds);
     if (want && (uint64_t)p.block * batch >= (1ull << 31))
         want = false;
     if (want) {
-        size_t budget = std::min<size_t>(r->max_lds, (size_t)r->cache_kb << 10);
+        // (256 bytes stay free for the batched kernel's static LDS)
+        size_t budget = std::min<size_t>(r->max_lds - 256, (size_t)sh.cache_kb << 10);
         size_t avail = budget > base ? budget - base : 0;
         int log2s = 0;
         while (((size_t)2 << log2s) * slot_bytes <= avail && log2s < 30)
@@ -414,64 +437,88 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
     p.cache = want;
     if (p.cache) {
         p.lds = base + (size_t)p.slots * slot_bytes;
+        uint64_t rr = std::min<uint64_t>(rounds, ((1ull << 31) - 1) / ((uint64_t)p.block * batch));
+        p.task_walkers = (uint64_t)p.block * std::max<uint64_t>(rr, 1);
         int per_cu = std::max<int>(1, (int)(r->lds_per_cu / std::max<size_t>(p.lds, 1)));
-        uint64_t lanes = (uint64_t)r->cu_count * (uint64_t)per_cu * (uint64_t)p.block;
-        uint64_t grid64 = std::min<uint64_t>((walkers + p.block - 1) / p.block,
-                                             lanes / (uint64_t)p.block);
-        p.grid = (int)std::max<uint64_t>(grid64, 1);
-        // walkers per launch: each lane takes at most `rounds` walkers, so a
-        // workgroup adds at most rounds * block * batch < 2^31 to one cell
-        uint64_t rounds = std::max<uint64_t>(1, ((1ull << 31) - 1) / ((uint64_t)p.block * batch));
-        p.walkers_per_launch = rounds * (uint64_t)p.grid * (uint64_t)p.block;
+        uint64_t tasks = (walkers + p.task_walkers - 1) / p.task_walkers;
+        p.grid = (int)std::max<uint64_t>(
+            1, std::min<uint64_t>(tasks, (uint64_t)r->cu_count * (uint64_t)per_cu));
     } else {
         p.block = walkers >= (uint64_t)r->cu_count * 256 ? 256 : 64;
-        if (r->block_pref > 0 && r->block_pref <= 256)
-            p.block = r->block_pref;
-        uint64_t grid64 = (walkers + (uint64_t)p.block - 1) / (uint64_t)p.block;
-        p.grid = (int)std::min<uint64_t>(grid64, 1u << 30);
-        p.lds = r->lds_bytes;
-        p.walkers_per_launch = walkers;
+        if (sh.block > 0 && sh.block <= 256)
+            p.block = sh.block;
+        p.task_walkers = (uint64_t)p.block * rounds;
+        uint64_t tasks = (walkers + p.task_walkers - 1) / p.task_walkers;
+        p.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(tasks, 1u << 30));
+        p.lds = base;
     }
     return p;
 }
 
 template <typename R>
-static int launch_chunk(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
-                        uint64_t walker0, uint64_t n_walkers, uint64_t batch, uint64_t last)
+static void walk_args(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
+                      uint64_t walker0, uint64_t n_walkers, uint64_t batch, uint64_t last,
+                      WalkArgs<R> &a)
 {
-    WalkArgs<R> a;
     fill_args(r, d, a);
     a.walker_base = walker0;
     a.n_walkers = n_walkers;
     a.batch = batch;
     a.last_samples = last;
+    a.task_walkers = p.task_walkers;
     a.cache_slots = p.slots;
     a.cache_log2 = p.log2s;
     a.cache_ways = p.ways;
     a.cache_direct = p.direct;
-    const int dims = r->spec.dims, nc = r->spec.color_dims;
-    int grid = (int)std::min<uint64_t>((uint64_t)p.grid, (n_walkers + p.block - 1) / p.block);
-    if constexpr (std::is_same_v<R, double>) {
-        return dims == 1 ? launch_walk_f64u64_d1(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
-             : dims == 2 ? launch_walk_f64u64_d2(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
-                         : launch_walk_f64u64_d3(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream);
-    } else if (r->mode == MODE_F32_U32) {
-        return dims == 1 ? launch_walk_f32u32_d1(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
-             : dims == 2 ? launch_walk_f32u32_d2(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
-                         : launch_walk_f32u32_d3(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream);
-    } else {
-        return dims == 1 ? launch_walk_f32u64_d1(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
-             : dims == 2 ? launch_walk_f32u64_d2(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream)
-                         : launch_walk_f32u64_d3(nc, r->vs, p.cache, a, grid, p.block, p.lds, r->stream);
+}
+
+static int launch_walk_any(int mode, int dims, const WalkLaunch &L)
+{
+    switch (mode) {
+    case MODE_F64_U64:
+        return dims == 1 ? launch_walk_f64u64_d1(L) : dims == 2 ? launch_walk_f64u64_d2(L)
+                                                                 : launch_walk_f64u64_d3(L);
+    case MODE_F32_U32:
+        return dims == 1 ? launch_walk_f32u32_d1(L) : dims == 2 ? launch_walk_f32u32_d2(L)
+                                                                 : launch_walk_f32u32_d3(L);
+    default:
+        return dims == 1 ? launch_walk_f32u64_d1(L) : dims == 2 ? launch_walk_f32u64_d2(L)
+                                                                 : launch_walk_f32u64_d3(L);
     }
+}
+
+template <typename R>
+static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
+                         uint64_t walker0, uint64_t walkers, uint64_t batch, uint64_t last)
+{
+    WalkArgs<R> a;
+    walk_args(r, d, p, walker0, walkers, batch, last, a);
+    WalkLaunch L;
+    L.nc = r->spec.color_dims;
+    L.vs = r->vs;
+    L.cache = p.cache;
+    L.grid = p.grid;
+    L.block = p.block;
+    L.lds = p.lds;
+    L.stream = r->stream;
+    L.args = &a;
+    return launch_walk_any(r->mode, r->spec.dims, L);
+}
+
+// walkers and samples per walker of a render of `samples`
+static void split_samples(const ffr_renderer *r, uint64_t samples, uint64_t &batch,
+                          uint64_t &walkers, uint64_t &last)
+{
+    batch = r->opt.batch_size ? r->opt.batch_size : ffr_auto_batch(r, samples);
+    walkers = (samples + batch - 1) / batch;
+    last = samples - (walkers - 1) * batch;
 }
 
 static int launch(ffr_renderer *r, uint64_t samples)
 {
-    uint64_t batch = r->opt.batch_size ? r->opt.batch_size : ffr_auto_batch(r, samples);
-    uint64_t walkers = (samples + batch - 1) / batch;
-    uint64_t last = samples - (walkers - 1) * batch;
-    LaunchPlan p = plan_launch(r, walkers, batch);
+    uint64_t batch, walkers, last;
+    split_samples(r, samples, batch, walkers, last);
+    LaunchPlan p = plan_launch(r, walkers, batch, renderer_shape(r));
     r->last_walkers = walkers;
     r->last_batch = batch;
     r->last_cache = p.cache ? p.slots : 0;
@@ -479,19 +526,16 @@ static int launch(ffr_renderer *r, uint64_t samples)
     if (rc)
         return fail(FFR_E_DEVICE, std::string("stats reset: ") + hipGetErrorString((hipError_t)rc));
     HIP_TRY(hipEventRecord(r->ev0, r->stream));
-    for (uint64_t w0 = 0; w0 < walkers; w0 += p.walkers_per_launch) {
-        uint64_t n = std::min(p.walkers_per_launch, walkers - w0);
-        uint64_t chunk_last = (w0 + n == walkers) ? last : batch;
-        rc = r->mode == MODE_F64_U64
-                 ? launch_chunk(r, r->f64, p, r->next_walker + w0, n, batch, chunk_last)
-                 : launch_chunk(r, r->f32, p, r->next_walker + w0, n, batch, chunk_last);
-        if (rc)
-            return fail(FFR_E_DEVICE, std::string("walker kernel launch: ") +
-                                          hipGetErrorString((hipError_t)rc));
-    }
+    rc = r->mode == MODE_F64_U64
+             ? launch_single(r, r->f64, p, r->next_walker, walkers, batch, last)
+             : launch_single(r, r->f32, p, r->next_walker, walkers, batch, last);
+    if (rc)
+        return fail(FFR_E_DEVICE, std::string("walker kernel launch: ") +
+                                      hipGetErrorString((hipError_t)rc));
     HIP_TRY(hipEventRecord(r->ev1, r->stream));
     r->next_walker += walkers;
     r->pending = true;
+    r->timed = true;
     return FFR_OK;
 }
 
@@ -505,9 +549,11 @@ static int finish(ffr_renderer *r, ffr_stats *out)
         std::vector<unsigned long long> xf((size_t)std::max(r->nxf, 1));
         HIP_TRY(hipMemcpy(xf.data(), r->dxf, sizeof(unsigned long long) * (size_t)r->nxf,
                           hipMemcpyDeviceToHost));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-        r->last_ms = ms;
+        if (r->timed) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+            r->last_ms = ms;
+        }
         r->s_iter += st.s_iter;
         r->s_plot += st.s_plot;
         for (int k = 0; k < r->nxf; ++k)
@@ -767,3 +813,5 @@ int ffr_xform_order(const ffr_renderer *r, int32_t *ids, double *cw, size_t n)
 }
 
 } // extern "C"
+
+#include "ffr_batch.inc"

@@ -20,6 +20,31 @@ __global__ void reset_stats_kernel(DevStats *st, unsigned long long *xf_count, i
         xf_count[k] = 0;
 }
 
+// the statistics of several renderers and the task-queue counters of a
+// batched launch: block b clears entry b, block 0 also the counters
+__global__ void reset_many_kernel(const ResetEntry *e, int n, unsigned *queues, int nq)
+{
+    const int b = blockIdx.x;
+    if (b < n) {
+        DevStats *st = e[b].st;
+        if (threadIdx.x == 0) {
+            st->s_iter = 0;
+            st->s_plot = 0;
+            st->n_bad = 0;
+            st->pad = 0;
+            for (int i = 0; i < 3; ++i) {
+                st->pmin[i] = (double)INFINITY;
+                st->pmax[i] = -(double)INFINITY;
+            }
+        }
+        for (int k = threadIdx.x; k < e[b].nxf; k += blockDim.x)
+            e[b].xf[k] = 0;
+    }
+    if (b == 0)
+        for (int i = threadIdx.x; i < nq; i += blockDim.x)
+            queues[i] = 0;
+}
+
 // addBuffer (buffer_renderer.hpp:375-392): word 0 of a cell adds as an integer,
 // the colour words add as floating point
 template <typename CT, typename CR>
@@ -88,6 +113,13 @@ static int grid_for(unsigned long long n, int block)
 int launch_reset_stats(DevStats *st, unsigned long long *xf_count, int nxf, hipStream_t s)
 {
     hipLaunchKernelGGL(reset_stats_kernel, dim3(1), dim3(256), 0, s, st, xf_count, nxf);
+    return (int)hipGetLastError();
+}
+
+int launch_reset_many(const ResetEntry *e, int n, unsigned *queues, int nq, hipStream_t s)
+{
+    hipLaunchKernelGGL(reset_many_kernel, dim3(n > 0 ? n : 1), dim3(256), 0, s, e, n, queues,
+                       nq);
     return (int)hipGetLastError();
 }
 

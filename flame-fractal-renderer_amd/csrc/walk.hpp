@@ -38,6 +38,7 @@ template <typename R> struct WalkArgs {
     uint64_t last_samples;     // samples of the last walker
     uint64_t bv_limit;
     uint64_t bv_cap;
+    uint64_t task_walkers;     // walkers per workgroup task (one flame's launch)
     int nxf, n_slots, has_final, fin_nv, settle, r;
     int tab_reals;
     int xf0, fin0;
@@ -58,6 +59,37 @@ template <typename R> struct WalkArgs {
     // xform record layout (flame_model.hpp XformLayout)
     int l_preA, l_preb, l_postA, l_postb, l_speed, l_color, l_has_color, l_has_pre,
         l_has_post, l_stride;
+};
+
+// one renderer's statistics, for the reset of a batched launch
+struct ResetEntry {
+    DevStats *st;
+    unsigned long long *xf;
+    int nxf;
+    int pad;
+};
+
+// one task of a batched launch: walkers [w_begin, w_end) of flame `flame`
+struct BatchTask {
+    unsigned flame;
+    unsigned pad;
+    uint64_t w_begin, w_end;
+};
+
+// a walker-kernel launch: one flame (args, a host WalkArgs) or a batch
+// (descs, device WalkArgs[] indexed by the tasks' flame numbers)
+struct WalkLaunch {
+    int nc = 0, vs = 0;
+    bool cache = false;
+    int grid = 0, block = 0;
+    size_t lds = 0;
+    hipStream_t stream = nullptr;
+    const void *args = nullptr;
+    const void *descs = nullptr;
+    const BatchTask *tasks = nullptr;
+    const unsigned *ntasks = nullptr; // device word: the number of tasks
+    unsigned *queue = nullptr;        // device word: next task (zero at launch)
+    bool prepare_only = false;        // set the kernel's LDS limit, launch nothing
 };
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;

@@ -2,9 +2,8 @@
 #include "walk_kernel.inc"
 
 namespace ffr {
-int launch_walk_f32u64_d1(int nc, int vs, bool cache, const WalkArgs<float> &a, int grid, int block,
-                          size_t lds, hipStream_t s)
+int launch_walk_f32u64_d1(const WalkLaunch &L)
 {
-    return launch_nc<float, unsigned long long, double, 1>(nc, vs, cache, a, grid, block, lds, s);
+    return launch_walk<float, unsigned long long, double, 1>(L);
 }
 } // namespace ffr

@@ -61,6 +61,16 @@ class Stats(C.Structure):
     ]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("samples", C.c_uint64),
+        ("tasks", C.c_uint64),
+        ("launches", C.c_uint32),
+        ("graph", C.c_uint32),
+    ]
+
+
 # every symbol include/ffr.h declares
 EXPORTS = [
     "ffr_default_options", "ffr_create", "ffr_destroy", "ffr_last_error",
@@ -70,6 +80,8 @@ EXPORTS = [
     "ffr_add_buffer", "ffr_write_buffer", "ffr_clear_buffer",
     "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
     "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
+    "ffr_batch_create", "ffr_batch_destroy", "ffr_batch_render", "ffr_batch_tune",
+    "ffr_batch_set_graph", "ffr_batch_kinds", "ffr_batch_shape",
 ]
 
 _lib = None
@@ -125,6 +137,16 @@ def lib():
     L.ffr_check_flame.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32,
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                   C.c_size_t, C.POINTER(C.c_size_t)]
+    L.ffr_batch_create.argtypes = [C.POINTER(P), C.c_size_t, C.POINTER(P)]
+    L.ffr_batch_destroy.argtypes = [P]
+    L.ffr_batch_destroy.restype = None
+    L.ffr_batch_render.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_int),
+                                   C.POINTER(BatchStats)]
+    L.ffr_batch_tune.argtypes = [P, C.POINTER(C.c_uint64), C.c_double]
+    L.ffr_batch_set_graph.argtypes = [P, C.c_int]
+    L.ffr_batch_kinds.restype = C.c_size_t
+    L.ffr_batch_kinds.argtypes = [P]
+    L.ffr_batch_shape.argtypes = [P, C.c_size_t, C.POINTER(C.c_int32)]
     for name in EXPORTS:
         if not hasattr(L, name):
             raise RuntimeError(f"libffr_hip.so does not export {name}")
@@ -309,3 +331,63 @@ class BufferRenderer:
             if lib().ffr_xform_order(self._h, ids, cw, m) == FFR_OK:
                 return [ids[i] for i in range(m)], [cw[i] for i in range(m)]
         raise FfrError(FFR_E_SIZE, "no xforms")
+
+
+class Batch:
+    """Several BufferRenderers rendered together: one persistent launch per
+    kernel kind, replayed as a HIP graph.  Each renderer renders exactly
+    what its own render() would with the same samples.  No counterpart in
+    the reference, which renders one flame per process."""
+
+    def __init__(self, renderers):
+        self.renderers = list(renderers)
+        arr = (C.c_void_p * len(self.renderers))(*[r._h for r in self.renderers])
+        h = C.c_void_p()
+        _check(lib().ffr_batch_create(arr, len(self.renderers), C.byref(h)))
+        self._h = h
+        self.last_stats = BatchStats()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ffr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _samples(self, samples):
+        if len(samples) != len(self.renderers):
+            raise ValueError("one sample count per renderer")
+        return (C.c_uint64 * len(samples))(*[int(s) for s in samples])
+
+    def render(self, samples) -> list[bool]:
+        """Render samples[i] on renderer i; per renderer, False when its
+        bad-value limit stopped it."""
+        st = (C.c_int * len(self.renderers))()
+        rc = lib().ffr_batch_render(self._h, self._samples(samples), st,
+                                    C.byref(self.last_stats))
+        if rc not in (FFR_OK, FFR_E_BADVALUE_LIMIT):
+            _check(rc)
+        for r in self.renderers:
+            lib().ffr_sync(r._h, C.byref(r.last_stats))
+        return [v == FFR_OK for v in st]
+
+    def tune(self, samples, fraction: float = 1.0 / 32):
+        _check(lib().ffr_batch_tune(self._h, self._samples(samples), float(fraction)))
+
+    def set_graph(self, on: bool):
+        _check(lib().ffr_batch_set_graph(self._h, 1 if on else 0))
+
+    def shapes(self):
+        out = []
+        for k in range(lib().ffr_batch_kinds(self._h)):
+            v = (C.c_int32 * 4)()
+            _check(lib().ffr_batch_shape(self._h, k, v))
+            out.append(tuple(v))
+        return out

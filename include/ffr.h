@@ -151,6 +151,47 @@ uint64_t ffr_auto_batch(const ffr_renderer *r, uint64_t samples);
 int ffr_check_flame(const char *flame_json, size_t len, uint32_t precision, int32_t *ids,
                     double *cum_weights, size_t cap, size_t *n_xforms);
 
+/*
+ * Batched rendering.  The reference renders one flame per BufferRenderer
+ * and one render() at a time (buffer_renderer.hpp:269-338); a batch renders
+ * many renderers' flames together: one persistent launch per kernel kind
+ * (precision, dimensions, colour words, variation set), whose workgroups
+ * take (flame, walker range) tasks from a queue, captured once in a HIP
+ * graph and replayed.  Each renderer renders exactly what ffr_render would
+ * with the same samples: the same walkers on the same streams into its own
+ * buffer, with its own statistics.  The renderers stay owned by the caller
+ * and must outlive the batch; they must be on one device and have no render
+ * pending.
+ */
+typedef struct ffr_batch ffr_batch;
+
+typedef struct ffr_batch_stats {
+    double kernel_ms;    /* device time of the batch (HIP events around it)      */
+    uint64_t samples;    /* samples rendered                                     */
+    uint64_t tasks;      /* workgroup tasks                                      */
+    uint32_t launches;   /* walker-kernel launches (kernel kinds)                */
+    uint32_t graph;      /* 1 when the launches ran as a replayed HIP graph      */
+} ffr_batch_stats;
+
+int ffr_batch_create(ffr_renderer *const *renderers, size_t n, ffr_batch **out);
+void ffr_batch_destroy(ffr_batch *b);
+/* render samples[i] on renderer i (0: none) and wait.  status (may be NULL)
+ * receives FFR_OK or FFR_E_BADVALUE_LIMIT per renderer; the call returns
+ * FFR_E_BADVALUE_LIMIT when any renderer stopped at its limit */
+int ffr_batch_render(ffr_batch *b, const uint64_t *samples, int *status,
+                     ffr_batch_stats *stats);
+/* the launch-shape tuner: times candidate shapes (lanes per workgroup, LDS
+ * cell-cache size, walkers per task) on `fraction` of each renderer's
+ * walkers, rendering into scratch memory, and keeps the fastest shape per
+ * kernel kind.  Buffers, statistics and stream positions are untouched. */
+int ffr_batch_tune(ffr_batch *b, const uint64_t *samples, double fraction);
+/* run the launches as a HIP graph (on, the default) or as plain launches */
+int ffr_batch_set_graph(ffr_batch *b, int on);
+/* the shape chosen for kernel kind k (k < ffr_batch_kinds): cache -1/0/1,
+ * cache KiB, lanes per workgroup, walkers per lane per task */
+size_t ffr_batch_kinds(const ffr_batch *b);
+int ffr_batch_shape(const ffr_batch *b, size_t k, int32_t shape[4]);
+
 #ifdef __cplusplus
 }
 #endif
