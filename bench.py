@@ -13,7 +13,11 @@ Also measured (sub-objects of `workloads`):
   - the Sierpinski triangle, 1024x1024 uint64 counts, 2^26 samples from
     1024 walkers of 65536 with 20 settle iterations, FP64 and FP32;
   - the 3-D Sierpinski tetrahedron, 512^3 uint32 counts, 2^33 samples,
-    FP32 trajectories.
+    FP32 trajectories;
+  - a batch of 256 seeded random colour flames (flames/batch: 2-8 xforms,
+    256^2 to 2048^2 cells, 1000 samples per cell, 3.6e11 samples in all)
+    rendered in one batched pass (persistent launch, HIP graph, launch shape
+    from the tuner), FP32 and FP64, with the per-flame L1 between them.
 
 Prints one JSON line on stdout.  `roofline` is for the walker kernel:
 algorithmic bytes = plotted samples x the bytes each plot adds into its
@@ -52,6 +56,10 @@ C2_SAMPLES = 1 << 32
 C1_SAMPLES = 1 << 26
 C3_SAMPLES = 1 << 33
 C2_SEED, C1_SEED, C3_SEED = 0x5EED0002, 0x5EED0001, 0x5EED0003
+C4_DIR = os.path.join(FLAMES, "batch")
+C4_SEED = 0x5EED0004
+C4_SAMPLES_PER_CELL = 1000
+C4_TUNE_FRACTION = 1.0 / 128
 HBM_PEAK_GBS = 8000.0
 # walker kernel HBM bytes per launch on the headline workload (FP32):
 # FETCH_SIZE 521.5 KiB x 2.0 + WRITE_SIZE 18455193.9 KiB x 1.0 per dispatch.
@@ -122,6 +130,70 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
     return out, hist
 
 
+def read_batch_flames():
+    import re
+    texts = []
+    for name in sorted(os.listdir(C4_DIR)):
+        if name.endswith(".json"):
+            # the files open with a // comment line, which the renderer accepts
+            texts.append(read(os.path.join(C4_DIR, name)))
+    cells = []
+    for t in texts:
+        d = json.loads(re.sub(r"^\s*//.*$", "", t, flags=re.M))
+        cells.append(d["size"][0] * d["size"][1])
+    return texts, cells
+
+
+def run_batch(ffr, texts, samples, precision, keep):
+    """One batched pass over all flames (after a tuning pass); returns the
+    rates and, when keep, each flame's normalised histogram.  `precision` is
+    one name for all flames or a list of one per flame."""
+    precs = precision if isinstance(precision, list) else [precision] * len(texts)
+    rs = [ffr.BufferRenderer(t, precision=p, seed=C4_SEED + i)
+          for i, (t, p) in enumerate(zip(texts, precs))]
+    b = ffr.Batch(rs)
+    t0 = time.perf_counter()
+    b.tune(samples, C4_TUNE_FRACTION)
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ok = b.render(samples)
+    wall = time.perf_counter() - t0
+    if not all(ok):
+        raise RuntimeError("a batched flame stopped at the bad-value limit")
+    st = b.last_stats
+    plotted = sum(r.samples_plotted for r in rs)
+    hists = None
+    if keep:
+        hists = []
+        for r in rs:
+            h = r.counts().astype("float64")
+            hists.append(h / max(h.sum(), 1.0))
+    out = {
+        "value": sum(samples) / wall,
+        "ms_per_step": 1e3 * wall,
+        "kernel_ms": st.kernel_ms,
+        "flames": len(rs),
+        "samples": int(sum(samples)),
+        "tasks": int(st.tasks),
+        "launches": int(st.launches),
+        "graph": bool(st.graph),
+        "shapes": [list(x) for x in b.shapes()],
+        "tune_s": tune_s,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": plotted * 32 / (st.kernel_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": plotted * 32 / (st.kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+    b.close()
+    for r in rs:
+        r.close()
+    return out, hists
+
+
 def l1(a, b):
     import numpy as np
     return float(np.abs(a - b).sum())
@@ -171,7 +243,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["c1", "c2", "c3"], default=None,
+    ap.add_argument("--only", choices=["c1", "c2", "c3", "c4"], default=None,
                     help="run one workload only (profiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run kernels of known traffic (counter scaling)")
@@ -235,7 +307,55 @@ def main():
         workloads["tetrahedron_512_fp32"] = res
         log("tetrahedron_512_fp32", json.dumps(res))
 
+    if want("c4") and not args.fp32_only:
+        texts, cells = read_batch_flames()
+        samples = [C4_SAMPLES_PER_CELL * c for c in cells]
+        log(f"bench c4: {len(texts)} flames, {sum(samples)} samples in one batched pass")
+
+        def l1_summary(hs, ref):
+            d = [l1(a, b) for a, b in zip(hs, ref)]
+            return d, {"mean": sum(d) / len(d), "max": max(d),
+                       "over_1e-3": int(sum(1 for x in d if x > 1e-3))}
+
+        res64, h64 = run_batch(ffr, texts, samples, "fp64", keep=True)
+        log("batch_256_fp64", json.dumps(res64))
+        res32, h32 = run_batch(ffr, texts, samples, "fp32", keep=True)
+        d32, res32["l1_vs_fp64"] = l1_summary(h32, h64)
+        del h32
+        log("batch_256_fp32", json.dumps(res32))
+        # the precision policy: a short probe per flame on the same streams;
+        # FP64 where FP32 and FP64 drift apart
+        t0 = time.perf_counter()
+        policy, probe = [], []
+        for i, (t, n) in enumerate(zip(texts, samples)):
+            pr, dl = ffr.choose_precision(t, min(n // 16, 1 << 22), seed=C4_SEED + i)
+            policy.append(pr)
+            probe.append(dl)
+        probe_s = time.perf_counter() - t0
+        resp, hp = run_batch(ffr, texts, samples, policy, keep=True)
+        _, resp["l1_vs_fp64"] = l1_summary(hp, h64)
+        del hp, h64
+        flagged = [i for i, p in enumerate(policy) if p == "fp64"]
+        resp["policy"] = {
+            "fp64_flames": len(flagged), "probe_s": probe_s,
+            "probe_threshold": ffr.PROBE_L1_THRESHOLD,
+            "fp32_l1_of_fp32_flames_max": max([d32[i] for i in range(len(texts))
+                                               if policy[i] == "fp32"] or [0.0]),
+            "fp32_l1_over_1e-3_caught": int(sum(1 for i in flagged if d32[i] > 1e-3)),
+            "fp32_l1_over_1e-3": int(sum(1 for x in d32 if x > 1e-3)),
+        }
+        workloads["batch_256_fp32"] = res32
+        workloads["batch_256_fp64"] = res64
+        workloads["batch_256_policy"] = resp
+        log("batch_256_policy", json.dumps(resp))
+        c1cpu = workloads.get("sierpinski_1024_fp64", {}).get("cpu")
+        if c1cpu:
+            for k in ("batch_256_fp32", "batch_256_policy"):
+                workloads[k]["vs_cpu_sierpinski_1024"] = workloads[k]["value"] / c1cpu["value"]
+
     if fp32 is None:
+        if workloads:
+            print(json.dumps({"workloads": workloads}), flush=True)
         return 0
     cpu = None
     if not args.no_cpu and not args.fp32_only:

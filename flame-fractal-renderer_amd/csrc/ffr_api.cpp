@@ -736,6 +736,79 @@ int ffr_check_flame(const char *flame_json, size_t len, uint32_t precision, int3
     return FFR_OK;
 }
 
+int ffr_histogram_l1(ffr_renderer *a, ffr_renderer *b, double *l1)
+{
+    if (!a || !b || !l1)
+        return fail(FFR_E_ARG, "ffr_histogram_l1: null argument");
+    if (a->cells != b->cells || a->device != b->device)
+        return fail(FFR_E_SIZE, "ffr_histogram_l1: buffers of different shapes or devices");
+    uint64_t sa = 0, sb = 0, mn, mx;
+    int rc = ffr_histogram_stats(a, &sa, &mn, &mx);
+    if (rc)
+        return rc;
+    rc = ffr_histogram_stats(b, &sb, &mn, &mx);
+    if (rc)
+        return rc;
+    HIP_TRY(hipSetDevice(a->device));
+    double *d = reinterpret_cast<double *>(a->dhist); // 3 words of scratch
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY((hipError_t)launch_l1_counts(a->dbuf, (int)a->cell_words, (int)a->word_bytes, b->dbuf,
+                                         (int)b->cell_words, (int)b->word_bytes, a->cells,
+                                         sa ? 1.0 / (double)sa : 0.0, sb ? 1.0 / (double)sb : 0.0,
+                                         d, a->stream));
+    HIP_TRY(hipMemcpyAsync(l1, d, sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    return FFR_OK;
+}
+
+int ffr_choose_precision(const char *flame_json, size_t len, const ffr_options *opt,
+                         uint64_t probe_samples, double threshold, uint32_t *precision,
+                         double *l1)
+{
+    if (!flame_json || !precision)
+        return fail(FFR_E_ARG, "ffr_choose_precision: null argument");
+    ffr_options o;
+    if (opt)
+        o = *opt;
+    else
+        ffr_default_options(&o);
+    o.word = FFR_WORD64;
+    // both renders draw the same words at the same iteration: one settle
+    // count for both (the defaults differ between the precisions)
+    if (o.settle_iters < 0)
+        o.settle_iters = 53;
+    ffr_renderer *r32 = nullptr, *r64 = nullptr;
+    o.precision = FFR_FP32;
+    int rc = ffr_create(flame_json, len, &o, &r32);
+    if (rc == FFR_OK) {
+        o.precision = FFR_FP64;
+        rc = ffr_create(flame_json, len, &o, &r64);
+    }
+    int rc32 = FFR_OK, rc64 = FFR_OK;
+    if (rc == FFR_OK) {
+        rc32 = ffr_render(r32, probe_samples, nullptr);
+        rc64 = ffr_render(r64, probe_samples, nullptr);
+        if (rc32 != FFR_OK && rc32 != FFR_E_BADVALUE_LIMIT)
+            rc = rc32;
+        else if (rc64 != FFR_OK && rc64 != FFR_E_BADVALUE_LIMIT)
+            rc = rc64;
+    }
+    double d = 0.0;
+    if (rc == FFR_OK)
+        rc = ffr_histogram_l1(r32, r64, &d);
+    ffr_destroy(r32);
+    ffr_destroy(r64);
+    if (rc != FFR_OK)
+        return rc;
+    // an FP32 render stopped by bad values where FP64 was not: the FP32
+    // attractor escapes
+    const bool escapes = rc32 == FFR_E_BADVALUE_LIMIT && rc64 == FFR_OK;
+    *precision = (!escapes && d <= threshold) ? FFR_FP32 : FFR_FP64;
+    if (l1)
+        *l1 = d;
+    return FFR_OK;
+}
+
 int ffr_clear_buffer(ffr_renderer *r)
 {
     if (!r)

@@ -1,6 +1,8 @@
 // Buffer and statistics kernels used by the C ABI.
 #include "util_kernels.hpp"
 
+#include <algorithm>
+
 namespace ffr {
 
 __global__ void reset_stats_kernel(DevStats *st, unsigned long long *xf_count, int nxf)
@@ -88,6 +90,28 @@ __global__ void hist_stats_kernel(const CT *buf, unsigned long long cells, int c
     }
 }
 
+// L1 distance of two normalised count histograms with the same cells:
+// sum over c of |a_c * inv_a - b_c * inv_b|, in double
+template <typename CA, typename CB>
+__global__ void l1_counts_kernel(const CA *a, int cwa, const CB *b, int cwb,
+                                 unsigned long long cells, double inv_a, double inv_b,
+                                 double *out)
+{
+    double s = 0.0;
+    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         c < cells; c += (unsigned long long)gridDim.x * blockDim.x) {
+        const double d = (double)a[c * (unsigned long long)cwa] * inv_a -
+                         (double)b[c * (unsigned long long)cwb] * inv_b;
+        s += d < 0.0 ? -d : d;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0)
+        atomicAdd(out, s);
+}
+
+__global__ void zero_double_kernel(double *p) { *p = 0.0; }
+
 __global__ void hist_stats_init_kernel(unsigned long long *out)
 {
     out[0] = 0;
@@ -150,6 +174,30 @@ int launch_hist_stats(const void *buf, unsigned long long cells, int cw, int wor
     else
         hipLaunchKernelGGL((hist_stats_kernel<unsigned int>), dim3(grid), dim3(block), 0, s,
                            (const unsigned int *)buf, cells, cw, out);
+    return (int)hipGetLastError();
+}
+
+int launch_l1_counts(const void *a, int cwa, int wba, const void *b, int cwb, int wbb,
+                     unsigned long long cells, double inv_a, double inv_b, double *out,
+                     hipStream_t s)
+{
+    const int block = 256;
+    const int grid = std::min(grid_for(cells, block), 2048);
+    hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(1), 0, s, out);
+    using U8 = unsigned long long;
+    using U4 = unsigned int;
+    if (wba == 8 && wbb == 8)
+        hipLaunchKernelGGL((l1_counts_kernel<U8, U8>), dim3(grid), dim3(block), 0, s,
+                           (const U8 *)a, cwa, (const U8 *)b, cwb, cells, inv_a, inv_b, out);
+    else if (wba == 8)
+        hipLaunchKernelGGL((l1_counts_kernel<U8, U4>), dim3(grid), dim3(block), 0, s,
+                           (const U8 *)a, cwa, (const U4 *)b, cwb, cells, inv_a, inv_b, out);
+    else if (wbb == 8)
+        hipLaunchKernelGGL((l1_counts_kernel<U4, U8>), dim3(grid), dim3(block), 0, s,
+                           (const U4 *)a, cwa, (const U8 *)b, cwb, cells, inv_a, inv_b, out);
+    else
+        hipLaunchKernelGGL((l1_counts_kernel<U4, U4>), dim3(grid), dim3(block), 0, s,
+                           (const U4 *)a, cwa, (const U4 *)b, cwb, cells, inv_a, inv_b, out);
     return (int)hipGetLastError();
 }
 

@@ -12,6 +12,10 @@ int launch_add_cells(void *dst, const void *src, unsigned long long words, int c
                      int word_bytes, hipStream_t s);
 int launch_hist_stats(const void *buf, unsigned long long cells, int cw, int word_bytes,
                       unsigned long long *out, hipStream_t s);
+// L1 distance of two normalised count histograms (word bytes 4 or 8 each)
+int launch_l1_counts(const void *a, int cwa, int wba, const void *b, int cwb, int wbb,
+                     unsigned long long cells, double inv_a, double inv_b, double *out,
+                     hipStream_t s);
 // zero a buffer with a kernel (usable inside stream capture)
 int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s);
 

@@ -80,6 +80,7 @@ EXPORTS = [
     "ffr_add_buffer", "ffr_write_buffer", "ffr_clear_buffer",
     "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
     "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
+    "ffr_histogram_l1", "ffr_choose_precision",
     "ffr_batch_create", "ffr_batch_destroy", "ffr_batch_render", "ffr_batch_tune",
     "ffr_batch_set_graph", "ffr_batch_kinds", "ffr_batch_shape",
 ]
@@ -137,6 +138,10 @@ def lib():
     L.ffr_check_flame.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32,
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                   C.c_size_t, C.POINTER(C.c_size_t)]
+    L.ffr_histogram_l1.argtypes = [P, P, C.POINTER(C.c_double)]
+    L.ffr_choose_precision.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(Options), C.c_uint64,
+                                       C.c_double, C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_double)]
     L.ffr_batch_create.argtypes = [C.POINTER(P), C.c_size_t, C.POINTER(P)]
     L.ffr_batch_destroy.argtypes = [P]
     L.ffr_batch_destroy.restype = None
@@ -331,6 +336,35 @@ class BufferRenderer:
             if lib().ffr_xform_order(self._h, ids, cw, m) == FFR_OK:
                 return [ids[i] for i in range(m)], [cw[i] for i in range(m)]
         raise FfrError(FFR_E_SIZE, "no xforms")
+
+
+def histogram_l1(a: "BufferRenderer", b: "BufferRenderer") -> float:
+    """L1 distance of two renderers' normalised count histograms (device)."""
+    out = C.c_double()
+    _check(lib().ffr_histogram_l1(a._h, b._h, C.byref(out)))
+    return out.value
+
+
+# probe distance above which a flame keeps FP64 trajectories
+PROBE_L1_THRESHOLD = 1e-2
+
+
+def choose_precision(flame_json: str, probe_samples: int, seed: int = 0, batch_size: int = 0,
+                     settle: int = -1, threshold: float = PROBE_L1_THRESHOLD):
+    """The precision policy: ("fp32" | "fp64", probe L1).  FP32 unless the
+    FP32 and FP64 renders of a short probe on the same streams drift apart."""
+    L = lib()
+    o = Options()
+    L.ffr_default_options(C.byref(o))
+    o.seed = seed
+    o.batch_size = batch_size
+    o.settle_iters = settle
+    prec = C.c_uint32()
+    d = C.c_double()
+    b = flame_json.encode()
+    _check(L.ffr_choose_precision(b, len(b), C.byref(o), int(probe_samples), float(threshold),
+                                  C.byref(prec), C.byref(d)))
+    return ("fp32" if prec.value == FFR_FP32 else "fp64"), d.value
 
 
 class Batch:

@@ -152,6 +152,24 @@ int ffr_check_flame(const char *flame_json, size_t len, uint32_t precision, int3
                     double *cum_weights, size_t cap, size_t *n_xforms);
 
 /*
+ * Precision policy (no counterpart in the reference, which fixes num_t at
+ * build time, types.hpp:29).  ffr_histogram_l1: the L1 distance between
+ * two renderers' normalised count histograms (same cell count), computed
+ * on the device.  ffr_choose_precision: renders `probe_samples` samples of
+ * the flame with FP32 and with FP64 trajectories on the same walker streams
+ * (seed, batch size and settle from opt; a settle of -1 becomes 53 for
+ * both, so both draw the same words) and compares the histograms.  The
+ * trajectories of a contractive flame track each other to rounding, so the
+ * distance stays near zero; a flame whose FP32 attractor escapes or
+ * degenerates drifts apart.  *precision receives FFR_FP32 when the distance
+ * is at most `threshold`, else FFR_FP64; *l1 (may be NULL) the distance.
+ */
+int ffr_histogram_l1(ffr_renderer *a, ffr_renderer *b, double *l1);
+int ffr_choose_precision(const char *flame_json, size_t len, const ffr_options *opt,
+                         uint64_t probe_samples, double threshold, uint32_t *precision,
+                         double *l1);
+
+/*
  * Batched rendering.  The reference renders one flame per BufferRenderer
  * and one render() at a time (buffer_renderer.hpp:269-338); a batch renders
  * many renderers' flames together: one persistent launch per kernel kind

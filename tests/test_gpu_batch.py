@@ -149,3 +149,43 @@ def test_batch_of_many_small_flames(ffr_mod):
     b.close()
     for r in batched + alone:
         r.close()
+
+
+# ------------------------------------------------- precision policy
+
+def test_histogram_l1_matches_host(ffr_mod):
+    text = TF.color_flame(3)
+    a = ffr_mod.BufferRenderer(text, seed=1, batch_size=1000)
+    b = ffr_mod.BufferRenderer(text, precision="fp32", word=32, seed=2, batch_size=1000)
+    twin = ffr_mod.BufferRenderer(text, seed=1, batch_size=1000)
+    for r in (a, b, twin):
+        assert r.render(200_000)
+    assert ffr_mod.histogram_l1(a, twin) == 0.0
+    ca = a.counts().astype(np.float64)
+    cb = b.counts().astype(np.float64)
+    want = float(np.abs(ca / ca.sum() - cb / cb.sum()).sum())
+    got = ffr_mod.histogram_l1(a, b)
+    assert want > 0.01 and abs(got - want) <= 1e-12 * max(1.0, want)
+    for r in (a, b, twin):
+        r.close()
+
+
+def rotations_only(size=64):
+    """Two isometries (rotations by 1 and sqrt(2) radians): nothing contracts,
+    so FP32 and FP64 trajectories on the same streams drift apart."""
+    def rot(t):
+        c, s = float(np.cos(t)), float(np.sin(t))
+        return {"A": [[c, -s], [s, c]], "b": [0, 0]}
+    return json.dumps({"dimensions": 2, "size": [size, size], "bounds": [[-1.5, 1.5]] * 2,
+                       "xforms": [{"weight": 1, "variations": [{"name": "linear", "weight": 1}],
+                                   "pre_affine": rot(1.0)},
+                                  {"weight": 1, "variations": [{"name": "linear", "weight": 1}],
+                                   "pre_affine": rot(float(np.sqrt(2.0)))}]})
+
+
+def test_precision_policy(ffr_mod):
+    sier = read_flame("sierpinski_1024.json").replace("[1024, 1024]", "[64, 64]")
+    prec, d = ffr_mod.choose_precision(sier, 1 << 20, seed=3)
+    assert prec == "fp32" and d < 1e-3, d
+    prec, d = ffr_mod.choose_precision(rotations_only(), 1 << 20, seed=3)
+    assert prec == "fp64" and d > ffr_mod.PROBE_L1_THRESHOLD, d
