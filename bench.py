@@ -17,7 +17,12 @@ Also measured (sub-objects of `workloads`):
   - a batch of 256 seeded random colour flames (flames/batch: 2-8 xforms,
     256^2 to 2048^2 cells, 1000 samples per cell, 3.6e11 samples in all)
     rendered in one batched pass (persistent launch, HIP graph, launch shape
-    from the tuner), FP32 and FP64, with the per-flame L1 between them.
+    from the tuner), FP32, FP64 and under the precision policy, with the
+    per-flame L1 against FP64;
+  - serving: 200 jobs arriving as a Poisson process at 200 per second (90 %
+    from the batch set, 10 % 3-D 256^3 u32 jobs), 4 chunks each, submitted
+    to the render server at their arrival times; throughput and p50/p99 job
+    latency, FP32 and FP64.  `p99_latency_s` is the FP32 serving p99.
 
 Prints one JSON line on stdout.  `roofline` is for the walker kernel:
 algorithmic bytes = plotted samples x the bytes each plot adds into its
@@ -59,7 +64,18 @@ C2_SEED, C1_SEED, C3_SEED = 0x5EED0002, 0x5EED0001, 0x5EED0003
 C4_DIR = os.path.join(FLAMES, "batch")
 C4_SEED = 0x5EED0004
 C4_SAMPLES_PER_CELL = 1000
-C4_TUNE_FRACTION = 1.0 / 128
+C4_TUNE_FRACTION = 1.0 / 256
+# serving trace: Poisson arrivals, 90 % jobs drawn from the 256-flame batch
+# set (1000 samples per cell), 10 % 3-D jobs at 256^3 u32 (2^30 samples,
+# 64 per cell), 4 chunks each, 200 GB of job buffers at most
+C5_DIR = os.path.join(FLAMES, "serve3d")
+C5_JOBS = 200
+C5_RATE = 200.0
+C5_SEED = 5
+C5_3D_SHARE = 0.1
+C5_3D_SAMPLES = 1 << 30
+C5_CHUNKS = 4
+C5_BUDGET = 200_000_000_000
 HBM_PEAK_GBS = 8000.0
 # walker kernel HBM bytes per launch on the headline workload (FP32):
 # FETCH_SIZE 521.5 KiB x 2.0 + WRITE_SIZE 18455193.9 KiB x 1.0 per dispatch.
@@ -79,6 +95,12 @@ def log(*a):
 def read(path):
     with open(path) as fh:
         return fh.read()
+
+
+# FP32 renders compared with FP64 ones take the FP64 settle count, so both
+# draw the same words on every iteration and only the precision differs
+# (the reference's defaults are 53 for double and 24 for float)
+SETTLE_FP64 = 53
 
 
 def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, seed,
@@ -149,7 +171,7 @@ def run_batch(ffr, texts, samples, precision, keep):
     rates and, when keep, each flame's normalised histogram.  `precision` is
     one name for all flames or a list of one per flame."""
     precs = precision if isinstance(precision, list) else [precision] * len(texts)
-    rs = [ffr.BufferRenderer(t, precision=p, seed=C4_SEED + i)
+    rs = [ffr.BufferRenderer(t, precision=p, seed=C4_SEED + i, settle=SETTLE_FP64)
           for i, (t, p) in enumerate(zip(texts, precs))]
     b = ffr.Batch(rs)
     t0 = time.perf_counter()
@@ -192,6 +214,70 @@ def run_batch(ffr, texts, samples, precision, keep):
     for r in rs:
         r.close()
     return out, hists
+
+
+def serving_trace(texts2d, cells2d):
+    import numpy as np
+    texts3d = [read(os.path.join(C5_DIR, n)) for n in sorted(os.listdir(C5_DIR))
+               if n.endswith(".json")]
+    rng = np.random.default_rng(C5_SEED)
+    at = np.cumsum(rng.exponential(1.0 / C5_RATE, C5_JOBS))
+    jobs = []
+    for i in range(C5_JOBS):
+        if rng.uniform() < C5_3D_SHARE:
+            jobs.append((float(at[i]), texts3d[int(rng.integers(len(texts3d)))], C5_3D_SAMPLES,
+                         True))
+        else:
+            k = int(rng.integers(len(texts2d)))
+            jobs.append((float(at[i]), texts2d[k], C4_SAMPLES_PER_CELL * cells2d[k], False))
+    return jobs
+
+
+def run_serving(ffr, trace, precision):
+    """Submit the trace at its arrival times to a server (the client thread
+    sleeps until each arrival), wait for every job, report throughput and
+    job latency percentiles."""
+    import numpy as np
+    with ffr.Server(hbm_budget=C5_BUDGET) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for i, (at, text, n, is3d) in enumerate(trace):
+            delay = at - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            word = 32 if (is3d and precision == "fp32") else 64
+            ids.append(srv.submit(text, n, C5_CHUNKS, precision=precision, word=word,
+                                  seed=C5_SEED * 1000 + i,
+                                  settle=SETTLE_FP64 if precision == "fp32" else -1))
+        infos = []
+        for k, j in enumerate(ids):
+            infos.append(srv.wait(j, timeout=900))
+            if (k + 1) % 50 == 0:
+                log(f"serving {precision}: {k + 1}/{len(ids)} jobs done")
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        for j in ids:
+            srv.release(j)
+    if any(x.state != ffr.JOB_DONE for x in infos):
+        raise RuntimeError("a served job failed")
+    lat = np.array([x.done_s - x.submit_s for x in infos])
+    first = np.array([x.first_chunk_s - x.submit_s for x in infos])
+    total = sum(n for _, _, n, _ in trace)
+    return {
+        "value": total / wall,
+        "unit": "samples/s",
+        "jobs": len(trace),
+        "samples": int(total),
+        "wall_s": wall,
+        "arrivals_s": trace[-1][0],
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "first_chunk_p50_s": float(np.percentile(first, 50)),
+        "first_chunk_p99_s": float(np.percentile(first, 99)),
+        "passes": int(st.passes),
+        "device_busy_frac": st.busy_ms / 1e3 / wall,
+        "peak_buffer_bytes": int(st.peak_bytes),
+    }
 
 
 def l1(a, b):
@@ -243,7 +329,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=["c1", "c2", "c3", "c4"], default=None,
+    ap.add_argument("--only", choices=["c1", "c2", "c3", "c4", "c5"], default=None,
                     help="run one workload only (profiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run kernels of known traffic (counter scaling)")
@@ -268,7 +354,7 @@ def main():
     if want("c2"):
         log(f"bench c2: {C2_SAMPLES} samples/step, warmup {W}, steps {S}")
         fp32, h32 = run_gpu(ffr, c2_text, precision="fp32", samples=C2_SAMPLES, steps=S,
-                            warmup=W, bytes_per_plot=32, seed=C2_SEED,
+                            warmup=W, bytes_per_plot=32, seed=C2_SEED, settle=SETTLE_FP64,
                             keep=not args.fp32_only)
         fp32["roofline"]["traffic"] = C2_FP32_TRAFFIC_BYTES
         log("c2 fp32", json.dumps(fp32))
@@ -326,11 +412,8 @@ def main():
         # the precision policy: a short probe per flame on the same streams;
         # FP64 where FP32 and FP64 drift apart
         t0 = time.perf_counter()
-        policy, probe = [], []
-        for i, (t, n) in enumerate(zip(texts, samples)):
-            pr, dl = ffr.choose_precision(t, min(n // 16, 1 << 22), seed=C4_SEED + i)
-            policy.append(pr)
-            probe.append(dl)
+        policy, probe = ffr.choose_precisions(texts, samples,
+                                              [C4_SEED + i for i in range(len(texts))])
         probe_s = time.perf_counter() - t0
         resp, hp = run_batch(ffr, texts, samples, policy, keep=True)
         _, resp["l1_vs_fp64"] = l1_summary(hp, h64)
@@ -338,11 +421,13 @@ def main():
         flagged = [i for i, p in enumerate(policy) if p == "fp64"]
         resp["policy"] = {
             "fp64_flames": len(flagged), "probe_s": probe_s,
-            "probe_threshold": ffr.PROBE_L1_THRESHOLD,
+            "target": ffr.POLICY_L1_TARGET, "probe_fraction": ffr.POLICY_PROBE_FRACTION,
             "fp32_l1_of_fp32_flames_max": max([d32[i] for i in range(len(texts))
                                                if policy[i] == "fp32"] or [0.0]),
             "fp32_l1_over_1e-3_caught": int(sum(1 for i in flagged if d32[i] > 1e-3)),
             "fp32_l1_over_1e-3": int(sum(1 for x in d32 if x > 1e-3)),
+            "probe_l1_max_fp32_flames": max([probe[i] for i in range(len(texts))
+                                             if policy[i] == "fp32"] or [0.0]),
         }
         workloads["batch_256_fp32"] = res32
         workloads["batch_256_fp64"] = res64
@@ -352,6 +437,15 @@ def main():
         if c1cpu:
             for k in ("batch_256_fp32", "batch_256_policy"):
                 workloads[k]["vs_cpu_sierpinski_1024"] = workloads[k]["value"] / c1cpu["value"]
+
+    if want("c5") and not args.fp32_only:
+        texts, cells = read_batch_flames()
+        trace = serving_trace(texts, cells)
+        log(f"bench c5: {len(trace)} jobs over {trace[-1][0]:.2f} s of arrivals")
+        for prec in ("fp32", "fp64"):
+            res = run_serving(ffr, trace, prec)
+            workloads[f"serving_{prec}"] = res
+            log(f"serving_{prec}", json.dumps(res))
 
     if fp32 is None:
         if workloads:
@@ -398,6 +492,7 @@ def main():
             "roofline": fp64["roofline"],
         },
         "l1_fp32_vs_fp64": l1_c2,
+        "p99_latency_s": workloads.get("serving_fp32", {}).get("latency_p99_s"),
         "workloads": workloads,
         "cpu_baseline": cpu,
     }

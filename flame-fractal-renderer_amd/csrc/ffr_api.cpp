@@ -8,7 +8,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -888,3 +895,4 @@ int ffr_xform_order(const ffr_renderer *r, int32_t *ids, double *cw, size_t n)
 } // extern "C"
 
 #include "ffr_batch.inc"
+#include "ffr_server.inc"

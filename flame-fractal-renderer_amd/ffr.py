@@ -71,6 +71,50 @@ class BatchStats(C.Structure):
     ]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_uint64),
+        ("pass_ms", C.c_double),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [
+        ("state", C.c_uint32),
+        ("chunks_done", C.c_uint32),
+        ("samples_done", C.c_uint64),
+        ("bytes", C.c_uint64),
+        ("batch_size", C.c_uint64),
+        ("submit_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("first_chunk_s", C.c_double),
+        ("done_s", C.c_double),
+        ("status", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("done", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("passes", C.c_uint64),
+        ("samples", C.c_uint64),
+        ("bytes_held", C.c_uint64),
+        ("peak_bytes", C.c_uint64),
+        ("queued", C.c_uint32),
+        ("running", C.c_uint32),
+        ("busy_ms", C.c_double),
+    ]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+FFR_E_TIMEOUT = 7
+
+
 # every symbol include/ffr.h declares
 EXPORTS = [
     "ffr_default_options", "ffr_create", "ffr_destroy", "ffr_last_error",
@@ -83,6 +127,9 @@ EXPORTS = [
     "ffr_histogram_l1", "ffr_choose_precision",
     "ffr_batch_create", "ffr_batch_destroy", "ffr_batch_render", "ffr_batch_tune",
     "ffr_batch_set_graph", "ffr_batch_kinds", "ffr_batch_shape",
+    "ffr_server_default_options", "ffr_server_create", "ffr_server_destroy",
+    "ffr_server_submit", "ffr_server_job", "ffr_server_job_error", "ffr_server_wait",
+    "ffr_server_read", "ffr_server_release", "ffr_server_get_stats",
 ]
 
 _lib = None
@@ -152,6 +199,20 @@ def lib():
     L.ffr_batch_kinds.restype = C.c_size_t
     L.ffr_batch_kinds.argtypes = [P]
     L.ffr_batch_shape.argtypes = [P, C.c_size_t, C.POINTER(C.c_int32)]
+    L.ffr_server_default_options.argtypes = [C.POINTER(ServerOptions)]
+    L.ffr_server_default_options.restype = None
+    L.ffr_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(P)]
+    L.ffr_server_destroy.argtypes = [P]
+    L.ffr_server_destroy.restype = None
+    L.ffr_server_submit.argtypes = [P, C.c_char_p, C.c_size_t, C.POINTER(Options), C.c_uint64,
+                                    C.c_uint32, C.c_void_p, C.c_size_t,
+                                    C.POINTER(C.c_uint64)]
+    L.ffr_server_job.argtypes = [P, C.c_uint64, C.POINTER(JobInfo)]
+    L.ffr_server_job_error.argtypes = [P, C.c_uint64, C.c_char_p, C.c_size_t]
+    L.ffr_server_wait.argtypes = [P, C.c_uint64, C.c_double]
+    L.ffr_server_read.argtypes = [P, C.c_uint64, C.c_void_p, C.c_size_t]
+    L.ffr_server_release.argtypes = [P, C.c_uint64]
+    L.ffr_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
     for name in EXPORTS:
         if not hasattr(L, name):
             raise RuntimeError(f"libffr_hip.so does not export {name}")
@@ -345,26 +406,68 @@ def histogram_l1(a: "BufferRenderer", b: "BufferRenderer") -> float:
     return out.value
 
 
-# probe distance above which a flame keeps FP64 trajectories
-PROBE_L1_THRESHOLD = 1e-2
+# the precision policy's bar: L1 between the FP32 and FP64 histograms of a
+# full render; the share of its samples the probe renders; the margin the
+# probe's estimate keeps below the bar (the estimate is itself a sample)
+POLICY_L1_TARGET = 1e-3
+POLICY_PROBE_FRACTION = 1.0 / 64
+POLICY_MARGIN = 0.5
 
 
-def choose_precision(flame_json: str, probe_samples: int, seed: int = 0, batch_size: int = 0,
-                     settle: int = -1, threshold: float = PROBE_L1_THRESHOLD):
-    """The precision policy: ("fp32" | "fp64", probe L1).  FP32 unless the
-    FP32 and FP64 renders of a short probe on the same streams drift apart."""
+def choose_precision(flame_json: str, samples: int, seed: int = 0, batch_size: int = 0,
+                     settle: int = -1, target: float = POLICY_L1_TARGET,
+                     fraction: float = POLICY_PROBE_FRACTION):
+    """The precision policy for a render of `samples` samples: ("fp32" |
+    "fp64", probe L1).  The probe renders `fraction` of the samples in FP32
+    and in FP64 on the same streams, with the walker length of the full
+    render (drift grows along a walker).  Where the trajectories drift
+    apart, the drifted samples are independent draws, and the L1 they add
+    scales as one over the square root of the sample count; so the probe's
+    bar is margin * target / sqrt(fraction)."""
     L = lib()
     o = Options()
     L.ffr_default_options(C.byref(o))
     o.seed = seed
-    o.batch_size = batch_size
+    o.batch_size = batch_size or L.ffr_auto_batch(None, int(samples))
     o.settle_iters = settle
+    probe = max(int(samples * fraction), int(o.batch_size))
+    threshold = POLICY_MARGIN * target / (min(1.0, probe / max(samples, 1)) ** 0.5)
     prec = C.c_uint32()
     d = C.c_double()
     b = flame_json.encode()
-    _check(L.ffr_choose_precision(b, len(b), C.byref(o), int(probe_samples), float(threshold),
+    _check(L.ffr_choose_precision(b, len(b), C.byref(o), probe, float(threshold),
                                   C.byref(prec), C.byref(d)))
     return ("fp32" if prec.value == FFR_FP32 else "fp64"), d.value
+
+
+def choose_precisions(flame_jsons, samples, seeds, target: float = POLICY_L1_TARGET,
+                      fraction: float = POLICY_PROBE_FRACTION, settle: int = 53):
+    """choose_precision for many flames at once: every probe pair (FP32 and
+    FP64 on the same streams) renders in one batched launch.  Returns
+    ([("fp32" | "fp64")], [probe L1])."""
+    L = lib()
+    rs, bars = [], []
+    try:
+        for text, n, seed in zip(flame_jsons, samples, seeds):
+            batch = int(L.ffr_auto_batch(None, int(n)))
+            probe = max(int(n * fraction), batch)
+            bars.append((probe, POLICY_MARGIN * target / (min(1.0, probe / max(n, 1)) ** 0.5)))
+            for prec in ("fp32", "fp64"):
+                rs.append(BufferRenderer(text, precision=prec, seed=seed, batch_size=batch,
+                                         settle=settle))
+        with Batch(rs) as b:
+            ok = b.render([p for p, _ in bars for _ in range(2)])
+        out, dist = [], []
+        for i, (_, bar) in enumerate(bars):
+            r32, r64 = rs[2 * i], rs[2 * i + 1]
+            d = histogram_l1(r32, r64)
+            escapes = (not ok[2 * i]) and ok[2 * i + 1]
+            out.append("fp32" if (d <= bar and not escapes) else "fp64")
+            dist.append(d)
+        return out, dist
+    finally:
+        for r in rs:
+            r.close()
 
 
 class Batch:
@@ -425,3 +528,91 @@ class Batch:
             _check(lib().ffr_batch_shape(self._h, k, v))
             out.append(tuple(v))
         return out
+
+
+class Server:
+    """A render server on one device (no counterpart in the reference):
+    jobs are admitted within an HBM budget, keep their buffers on the device
+    across `chunks` increments, and share batched passes run by a worker
+    thread; each renders what BufferRenderer.render would, chunk by chunk."""
+
+    def __init__(self, hbm_budget: int = 0, pass_ms: float = 0.0, device: int = 0):
+        L = lib()
+        o = ServerOptions()
+        L.ffr_server_default_options(C.byref(o))
+        if hbm_budget:
+            o.hbm_budget = hbm_budget
+        if pass_ms:
+            o.pass_ms = pass_ms
+        o.device = device
+        h = C.c_void_p()
+        _check(L.ffr_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ffr_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, flame_json: str, samples: int, chunks: int = 4, precision: str = "fp32",
+               word: int = 64, seed: int = 0, batch_size: int = 0, settle: int = -1,
+               bad_value_limit: int = 256, init: np.ndarray | None = None) -> int:
+        L = lib()
+        o = Options()
+        L.ffr_default_options(C.byref(o))
+        o.precision = FFR_FP32 if precision == "fp32" else FFR_FP64
+        o.word = FFR_WORD32 if word == 32 else FFR_WORD64
+        o.seed = seed
+        o.batch_size = batch_size
+        o.settle_iters = settle
+        o.bad_value_limit = bad_value_limit
+        b = flame_json.encode()
+        job = C.c_uint64()
+        if init is not None:
+            init = np.ascontiguousarray(init)
+            ptr, n = init.ctypes.data_as(C.c_void_p), init.nbytes
+        else:
+            ptr, n = None, 0
+        _check(L.ffr_server_submit(self._h, b, len(b), C.byref(o), int(samples), int(chunks),
+                                   ptr, n, C.byref(job)))
+        return job.value
+
+    def job(self, job: int) -> JobInfo:
+        info = JobInfo()
+        _check(lib().ffr_server_job(self._h, job, C.byref(info)))
+        return info
+
+    def job_error(self, job: int) -> str:
+        buf = C.create_string_buffer(1024)
+        _check(lib().ffr_server_job_error(self._h, job, buf, len(buf)))
+        return buf.value.decode()
+
+    def wait(self, job: int, timeout: float = 600.0) -> JobInfo:
+        rc = lib().ffr_server_wait(self._h, job, float(timeout))
+        if rc == FFR_E_TIMEOUT:
+            raise TimeoutError(f"job {job} did not finish in {timeout} s")
+        _check(rc)
+        return self.job(job)
+
+    def read(self, job: int, word: int = 64) -> np.ndarray:
+        info = self.job(job)
+        out = np.empty(info.bytes // (word // 8), dtype=np.uint64 if word == 64 else np.uint32)
+        _check(lib().ffr_server_read(self._h, job, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def release(self, job: int):
+        _check(lib().ffr_server_release(self._h, job))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().ffr_server_get_stats(self._h, C.byref(st)))
+        return st

@@ -11,6 +11,10 @@
       [0, 1]^3.
   batch/flame_NNNN.json         256 flames from the same generator with 2-8
       xforms (seeds 1000-1255), square sizes drawn from 256..2048.
+  serve3d/flame_NNNN.json       16 random 3-D linear flames (seeds 3000-3015):
+      4 xforms, affine maps with entries uniform in [-1, 1] rescaled to
+      spectral norm <= 0.6, Dirichlet(1) weights; 256^3 cells framed by the
+      0.5/99.5 percentile box of a chaos game.
 
 The bounds of a random flame frame its attractor: a short numpy chaos game
 (the generator's own, double precision) gives the 0.5/99.5 percentile box,
@@ -154,6 +158,46 @@ def batch_flames(n: int = 256, seed0: int = 1000):
     return out
 
 
+def random_3d_flame(seed: int, size: int = 256) -> dict:
+    rng = np.random.default_rng(seed)
+    n = 4
+    weights = rng.dirichlet(np.ones(n))
+    maps = []
+    for _ in range(n):
+        A = rng.uniform(-1.0, 1.0, size=(3, 3))
+        s = np.linalg.norm(A, 2)
+        if s > 0.6:
+            A *= 0.6 / s
+        maps.append((A, rng.uniform(-1.0, 1.0, size=3)))
+    # frame: a short chaos game in double precision
+    frng = np.random.default_rng(seed + 7919)
+    cw = np.cumsum(weights)
+    p = frng.uniform(-1, 1, (4096, 3))
+    pts = []
+    for it in range(32 + 128):
+        k = np.minimum(np.searchsorted(cw, frng.uniform(0, 1, len(p))), n - 1)
+        q = np.empty_like(p)
+        for j, (A, b) in enumerate(maps):
+            m = k == j
+            q[m] = p[m] @ A.T + b
+        p = q
+        if it >= 32:
+            pts.append(p.copy())
+    pts = np.concatenate(pts)
+    lo = np.percentile(pts, 0.5, axis=0)
+    hi = np.percentile(pts, 99.5, axis=0)
+    c, h = (lo + hi) / 2, 1.1 * (hi - lo) / 2
+    return {
+        "dimensions": 3,
+        "size": [size, size, size],
+        "bounds": [[float(c[i] - h[i]), float(c[i] + h[i])] for i in range(3)],
+        "xforms": [{"weight": float(w), "variations": [{"name": "linear", "weight": 1.0}],
+                    "pre_affine": {"A": [[float(v) for v in row] for row in A],
+                                   "b": [float(v) for v in b]}}
+                   for w, (A, b) in zip(weights, maps)],
+    }
+
+
 def _write(path, doc, header=None):
     text = json.dumps(doc, indent=1)
     if header:
@@ -173,6 +217,11 @@ def main():
     for s, doc in batch_flames():
         _write(os.path.join(bdir, f"flame_{s}.json"), doc,
                f"batch flame, seed {s} (gen_flames.py)")
+    sdir = os.path.join(HERE, "serve3d")
+    os.makedirs(sdir, exist_ok=True)
+    for s in range(3000, 3016):
+        _write(os.path.join(sdir, f"flame_{s}.json"), random_3d_flame(s),
+               f"3-D serving flame, seed {s} (gen_flames.py)")
 
 
 if __name__ == "__main__":

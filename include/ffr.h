@@ -33,7 +33,8 @@ enum {
     FFR_E_DEVICE = 3,           /* HIP runtime failure                         */
     FFR_E_BADVALUE_LIMIT = 4,   /* render stopped: render() returned false     */
     FFR_E_SIZE = 5,             /* buffer size mismatch (addBuffer/readBuffer) */
-    FFR_E_UNSUPPORTED = 6       /* flame outside this build's limits           */
+    FFR_E_UNSUPPORTED = 6,      /* flame outside this build's limits           */
+    FFR_E_TIMEOUT = 7           /* a wait ran out of time                      */
 };
 
 /* trajectory precision: num_t of the reference (src/types/types.hpp:29) */
@@ -139,7 +140,8 @@ int ffr_bad_values(const ffr_renderer *r, uint64_t *xform_ids, double *points,
  * and their cumulative weights as doubles */
 int ffr_xform_order(const ffr_renderer *r, int32_t *ids, double *cum_weights, size_t n);
 
-/* the batch size the renderer picks for `samples` when batch_size is 0 */
+/* the batch size the renderer picks for `samples` when batch_size is 0
+ * (the rule depends on the sample count only; r may be NULL) */
 uint64_t ffr_auto_batch(const ffr_renderer *r, uint64_t samples);
 
 /*
@@ -209,6 +211,79 @@ int ffr_batch_set_graph(ffr_batch *b, int on);
  * cache KiB, lanes per workgroup, walkers per lane per task */
 size_t ffr_batch_kinds(const ffr_batch *b);
 int ffr_batch_shape(const ffr_batch *b, size_t k, int32_t shape[4]);
+
+/*
+ * Serving (no counterpart in the reference, which renders one flame per
+ * process).  A server renders many jobs on one device.  A job is a flame,
+ * render options, a sample count split into `chunks` increments, and
+ * optionally a starting buffer that is added first (ffr-buf -i).  Jobs are
+ * admitted in arrival order while their buffers fit the HBM budget, and
+ * keep their buffers on the device until released.  A worker thread runs
+ * scheduling passes: each pass gives every running job a slice of walkers
+ * of its current chunk and renders all slices in one batched launch, so a
+ * job that arrives while others run joins at the next pass and shares the
+ * device with them.  A job renders exactly what ffr_render would render
+ * chunk by chunk with the same options: the same walkers on the same
+ * streams (a batch size of 0 in the options becomes the automatic one for
+ * the first chunk, fixed for the job).  The bad-value limit applies per
+ * pass; a job that reaches it fails with FFR_E_BADVALUE_LIMIT.  Calls may
+ * come from any host thread.
+ */
+typedef struct ffr_server ffr_server;
+
+enum { FFR_JOB_QUEUED = 0, FFR_JOB_RUNNING = 1, FFR_JOB_DONE = 2, FFR_JOB_FAILED = 3 };
+
+typedef struct ffr_server_options {
+    uint64_t hbm_budget;   /* bytes of job buffers held at once (0: 200e9)          */
+    double pass_ms;        /* target device time of one pass, ms (0: 40)            */
+    int32_t device;        /* HIP device ordinal                                    */
+    int32_t reserved;
+} ffr_server_options;
+
+typedef struct ffr_job_info {
+    uint32_t state;        /* FFR_JOB_*                                             */
+    uint32_t chunks_done;
+    uint64_t samples_done;
+    uint64_t bytes;        /* the job's buffer (ffr_buffer_bytes)                   */
+    uint64_t batch_size;   /* samples per walker of the job                         */
+    double submit_s;       /* seconds since the server started; -1 until it happens */
+    double admit_s;
+    double first_chunk_s;
+    double done_s;
+    int32_t status;        /* FFR_OK, or the error of a failed job                  */
+    int32_t reserved;
+} ffr_job_info;
+
+typedef struct ffr_server_stats {
+    uint64_t submitted, done, failed;
+    uint64_t passes;
+    uint64_t samples;      /* samples rendered                                      */
+    uint64_t bytes_held;   /* job buffers on the device now                         */
+    uint64_t peak_bytes;
+    uint32_t queued, running;
+    double busy_ms;        /* device time of all passes                             */
+} ffr_server_stats;
+
+void ffr_server_default_options(ffr_server_options *opt);
+/* starts the worker thread */
+int ffr_server_create(const ffr_server_options *opt, ffr_server **out);
+/* stops the worker after its current pass and frees every job */
+void ffr_server_destroy(ffr_server *s);
+/* validate the flame and queue the job; init (may be NULL) is a buffer of
+ * the job's layout added before rendering, init_bytes its size */
+int ffr_server_submit(ffr_server *s, const char *flame_json, size_t len, const ffr_options *opt,
+                      uint64_t samples, uint32_t chunks, const void *init, size_t init_bytes,
+                      uint64_t *job);
+int ffr_server_job(ffr_server *s, uint64_t job, ffr_job_info *info);
+/* the message of a failed job */
+int ffr_server_job_error(ffr_server *s, uint64_t job, char *buf, size_t cap);
+/* wait until the job is done or failed (FFR_E_TIMEOUT after timeout_s) */
+int ffr_server_wait(ffr_server *s, uint64_t job, double timeout_s);
+/* the job's buffer between passes: partial while it runs, final when done */
+int ffr_server_read(ffr_server *s, uint64_t job, void *dst, size_t bytes);
+/* free a done or failed job's buffer */
+int ffr_server_release(ffr_server *s, uint64_t job);
+int ffr_server_get_stats(ffr_server *s, ffr_server_stats *st);
 
 #ifdef __cplusplus
 }

@@ -170,22 +170,25 @@ def test_histogram_l1_matches_host(ffr_mod):
         r.close()
 
 
-def rotations_only(size=64):
-    """Two isometries (rotations by 1 and sqrt(2) radians): nothing contracts,
-    so FP32 and FP64 trajectories on the same streams drift apart."""
-    def rot(t):
-        c, s = float(np.cos(t)), float(np.sin(t))
+def chaotic(size=64):
+    """Swirl and sinusoidal after expanding rotations: nearby points separate,
+    so FP32 and FP64 trajectories on the same streams drift apart (the CPU
+    oracle puts the FP32/FP64 L1 of this flame at 0.06 for 2^20 samples)."""
+    def rot(t, k):
+        c, s = float(np.cos(t)) * k, float(np.sin(t)) * k
         return {"A": [[c, -s], [s, c]], "b": [0, 0]}
-    return json.dumps({"dimensions": 2, "size": [size, size], "bounds": [[-1.5, 1.5]] * 2,
-                       "xforms": [{"weight": 1, "variations": [{"name": "linear", "weight": 1}],
-                                   "pre_affine": rot(1.0)},
-                                  {"weight": 1, "variations": [{"name": "linear", "weight": 1}],
-                                   "pre_affine": rot(float(np.sqrt(2.0)))}]})
+    return json.dumps({"dimensions": 2, "size": [size, size], "bounds": [[-2, 2]] * 2,
+                       "xforms": [{"weight": 1, "variations": [{"name": "swirl", "weight": 1}],
+                                   "pre_affine": rot(0.5, 1.3)},
+                                  {"weight": 1, "variations": [{"name": "sinusoidal", "weight": 1}],
+                                   "pre_affine": rot(1.0, 2.0)}]})
 
 
 def test_precision_policy(ffr_mod):
     sier = read_flame("sierpinski_1024.json").replace("[1024, 1024]", "[64, 64]")
-    prec, d = ffr_mod.choose_precision(sier, 1 << 20, seed=3)
+    # renders of 2^26 samples: the probe takes 2^20 of them, its bar is
+    # 0.5 * 1e-3 * sqrt(64) = 4e-3
+    prec, d = ffr_mod.choose_precision(sier, 1 << 26, seed=3)
     assert prec == "fp32" and d < 1e-3, d
-    prec, d = ffr_mod.choose_precision(rotations_only(), 1 << 20, seed=3)
-    assert prec == "fp64" and d > ffr_mod.PROBE_L1_THRESHOLD, d
+    prec, d = ffr_mod.choose_precision(chaotic(), 1 << 26, seed=3)
+    assert prec == "fp64" and d > 4e-3, d
