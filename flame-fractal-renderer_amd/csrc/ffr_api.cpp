@@ -74,6 +74,8 @@ struct ffr_renderer {
     unsigned long long *dbv_xfs = nullptr;
     double *dbv_pts = nullptr;
     unsigned long long *dhist = nullptr;
+    void *dargs = nullptr;                // the walker kernel's arguments
+    std::vector<unsigned char> hargs;     // their host copy, alive until the copy ran
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -255,6 +257,7 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
     HIP_TRY_D(hipMalloc(&rp->dbv_xfs, sizeof(unsigned long long) * rp->opt.bad_value_cap));
     HIP_TRY_D(hipMalloc(&rp->dbv_pts, sizeof(double) * 3 * rp->opt.bad_value_cap));
     HIP_TRY_D(hipMalloc(&rp->dhist, sizeof(unsigned long long) * 3));
+    HIP_TRY_D(hipMalloc(&rp->dargs, std::max(sizeof(WalkArgs<double>), sizeof(WalkArgs<float>))));
     HIP_TRY_D((hipError_t)launch_fill_zero(rp->dbuf, bytes, rp->stream));
     HIP_TRY_D(hipStreamSynchronize(rp->stream));
 #undef HIP_TRY_D
@@ -285,6 +288,7 @@ void ffr_destroy(ffr_renderer *r)
     (void)hipFree(r->dbv_xfs);
     (void)hipFree(r->dbv_pts);
     (void)hipFree(r->dhist);
+    (void)hipFree(r->dargs);
     if (r->ev0)
         (void)hipEventDestroy(r->ev0);
     if (r->ev1)
@@ -352,11 +356,10 @@ static void fill_args(ffr_renderer *r, const DeviceFlame<R> &d, WalkArgs<R> &a)
         a.slot_type[s] = d.slot_type[s];
         a.slot_off[s] = d.slot_off[s];
         a.slot_stride[s] = d.slot_stride[s];
-        const int code = basic_code(d.slot_type[s]);
-        if (d.slot_type[s] >= 0 && code >= 0 && code < kBasicTypes &&
-            basic_type(code) == d.slot_type[s])
-            a.basic_slots[code] |= 1ull << s;
     }
+    a.bt0 = d.bt0;
+    a.bt_stride = d.bt_stride;
+    a.bt_nvmax = d.bt_nvmax;
     for (size_t j = 0; j < d.fin_type.size(); ++j) {
         a.fin_type[j] = d.fin_type[j];
         a.fin_off[j] = d.fin_off[j];
@@ -500,6 +503,12 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
 {
     WalkArgs<R> a;
     walk_args(r, d, p, walker0, walkers, batch, last, a);
+    r->hargs.resize(sizeof a);
+    std::memcpy(r->hargs.data(), &a, sizeof a);
+    hipError_t e = hipMemcpyAsync(r->dargs, r->hargs.data(), sizeof a, hipMemcpyHostToDevice,
+                                  r->stream);
+    if (e != hipSuccess)
+        return (int)e;
     WalkLaunch L;
     L.nc = r->spec.color_dims;
     L.vs = r->vs;
@@ -508,7 +517,7 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     L.block = p.block;
     L.lds = p.lds;
     L.stream = r->stream;
-    L.args = &a;
+    L.args = r->dargs;
     return launch_walk_any(r->mode, r->spec.dims, L);
 }
 

@@ -987,6 +987,42 @@ template <typename R> DeviceFlame<R> compile_flame(const FlameSpec &spec)
                 E[pofs + i] = tmp[i];
         }
     }
+    // per-xform variation lists (see DeviceFlame::bt0)
+    {
+        auto code_of = [](int type) {
+            switch (type) {
+            case 0: case 1: case 2: return type; // linear, sinusoidal, spherical
+            case 20: case 21: case 22: return type - 17; // swirl, horseshoe, polar
+            default: return -1;
+            }
+        };
+        bool basic = dims == 2;
+        int nvmax = 0;
+        for (int k = 0; k < d.nxf && basic; ++k) {
+            nvmax = std::max(nvmax, (int)lists[(size_t)k].size());
+            for (const VarSpec *v : lists[(size_t)k])
+                basic = basic && code_of(v->type) >= 0;
+        }
+        if (basic) {
+            d.bt_nvmax = nvmax;
+            d.bt_stride = 2 + 2 * nvmax;
+            d.bt0 = (int)T.size();
+            T.resize(T.size() + (size_t)(d.bt_stride * d.nxf), (R)0);
+            for (int k = 0; k < d.nxf; ++k) {
+                R *B = T.data() + d.bt0 + (size_t)(k * d.bt_stride);
+                int mask = 0;
+                const auto &L = lists[(size_t)k];
+                B[0] = (R)(int)L.size();
+                for (size_t j = 0; j < L.size(); ++j) {
+                    const int c = code_of(L[j]->type);
+                    mask |= 1 << c;
+                    B[2 + 2 * j] = (R)c;
+                    B[3 + 2 * j] = (R)L[j]->weight;
+                }
+                B[1] = (R)mask;
+            }
+        }
+    }
     T.resize(align2(T.size()), (R)0);
     if (ns > 64)
         throw JsonError("flame has " + std::to_string(ns) +

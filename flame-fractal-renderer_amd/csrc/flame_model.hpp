@@ -105,6 +105,11 @@ template <typename R> struct DeviceFlame {
     std::vector<int> slot_type, slot_off, slot_stride;
     std::vector<unsigned long long> masks; // [nxf] bit s set when the xform uses slot s
     std::vector<int> fin_type, fin_off;
+    // per-xform variation lists for 2-D flames whose variations are all of
+    // the basic six (linear, sinusoidal, spherical, swirl, horseshoe,
+    // polar; codes 0..5): [count, code mask, (code, weight) x bt_nvmax]
+    // per xform in JSON order, at table offset bt0 (-1: not built)
+    int bt0 = -1, bt_stride = 0, bt_nvmax = 0;
     uint32_t type_set[4] = {0, 0, 0, 0}; // bitset of variation types present
     int max_draws = 0;
 };

@@ -53,9 +53,8 @@ template <typename R> struct WalkArgs {
     int slot_stride[kMaxSlots]; // entry size
     int fin_type[kMaxFinalVars];
     int fin_off[kMaxFinalVars];
-    // kernels for the six basic variations in 2-D: the slots of each basic
-    // type (bit s set when slot s has that type), by basic_code
-    unsigned long long basic_slots[6];
+    // per-xform variation lists of all-basic 2-D flames (DeviceFlame::bt0)
+    int bt0, bt_stride, bt_nvmax, bt_pad;
     // xform record layout (flame_model.hpp XformLayout)
     int l_preA, l_preb, l_postA, l_postb, l_speed, l_color, l_has_color, l_has_pre,
         l_has_post, l_stride;
@@ -76,7 +75,7 @@ struct BatchTask {
     uint64_t w_begin, w_end;
 };
 
-// a walker-kernel launch: one flame (args, a host WalkArgs) or a batch
+// a walker-kernel launch: one flame (args, a device WalkArgs) or a batch
 // (descs, device WalkArgs[] indexed by the tasks' flame numbers)
 struct WalkLaunch {
     int nc = 0, vs = 0;
