@@ -22,10 +22,13 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 {
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo0 = 0xD2511F53u * c0;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t lo1 = 0xCD9E8D57u * c2;
+        // one 32x32->64 product gives both halves (v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32);
+        uint32_t lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32);
+        uint32_t lo1 = (uint32_t)p1;
         uint32_t n0 = hi1 ^ c1 ^ k0;
         uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
@@ -49,8 +52,9 @@ struct Stream {
 
     __device__ __forceinline__ void init(uint64_t seed, uint64_t walker)
     {
-        k0 = (uint32_t)seed;
-        k1 = (uint32_t)(seed >> 32);
+        // the key is the same for every lane: scalar registers, scalar adds
+        k0 = __builtin_amdgcn_readfirstlane((uint32_t)seed);
+        k1 = __builtin_amdgcn_readfirstlane((uint32_t)(seed >> 32));
         w0 = (uint32_t)walker;
         w1 = (uint32_t)(walker >> 32);
         n = 0;

@@ -75,6 +75,52 @@ template <typename R> __device__ __forceinline__ R dev_norm2sq(const R *x, int n
     return ret;
 }
 
+// The six basic variations of one 2-D point, for kernels that evaluate each
+// once per iteration: linear, sinusoidal, spherical, swirl, horseshoe,
+// polar (codes 0..5).  Each output is the expression calc2d / var_eval
+// evaluate for it (variations.hpp:169-208, 509-555), with the squared norm
+// and its root shared.  `need` has bit k set when some lane of the wave
+// uses code k; the others are left untouched.
+template <typename R>
+__device__ __forceinline__ void basic6(unsigned need, const R *t, R (&o)[6][2])
+{
+    const R EPS = Consts<R>::eps;
+    const R x = t[0], y = t[1];
+    const R n2sq = x * x + y * y;
+    o[0][0] = x;
+    o[0][1] = y;
+    if (need & 2u) {
+        o[1][0] = m_sin(x);
+        o[1][1] = m_sin(y);
+    }
+    if (need & 4u) {
+        R r = 1.0 / (n2sq + EPS);
+        o[2][0] = x * r;
+        o[2][1] = y * r;
+    }
+    if (need & 8u) {
+        R sr, cr;
+        m_sincos(n2sq, &sr, &cr);
+        o[3][0] = x * sr - y * cr;
+        o[3][1] = x * cr + y * sr;
+    }
+    R n2 = 0;
+    if (need & 48u)
+        n2 = (R)m_sqrt(n2sq);
+    if (need & 16u) {
+        R r = 1.0 / (n2 + EPS);
+        R a = (x - y) * (x + y);
+        R b = 2.0 * x * y;
+        o[4][0] = a * r;
+        o[4][1] = b * r;
+    }
+    if (need & 32u) {
+        R a = m_atan2(y, x);
+        o[5][0] = a * M_1_PI;
+        o[5][1] = n2 - 1.0;
+    }
+}
+
 // 2-D variations (variations.hpp:505-2376)
 template <typename R, int VS>
 __device__ __forceinline__ void calc2d(int type, const R *p, R x, R y, R &ox, R &oy, Stream &st)
