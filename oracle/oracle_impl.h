@@ -16,10 +16,12 @@
  *   render loop (init / settle / iterate / bad values / extremes / index)
  *       pinned by brute force on tiny grids, closed-form walkers (a constant
  *       map), the bad-value re-init schedule and the draw-count contract.
- *   variations: the six of the benchmark flames and the others are pinned at
- *       special points by closed forms (tests/test_oracle_variations.py);
- *       the remaining arithmetic is "parity unpinned" beyond those points
- *       (the reference cannot be built here without stand-in headers).
+ *   variations: every one of the 98 is pinned at one or more points where
+ *       the mathematics fixes the image (closed forms, or for variations that
+ *       draw, the words the stream holds), and by the draw count of each
+ *       (tests/test_oracle_variations.py).  Between those points the
+ *       arithmetic rests on the host libm and is "parity unpinned" (the
+ *       reference cannot be built here without stand-in headers).
  */
 
 #define ORC_CAT2(a, b) a##_##b
