@@ -75,6 +75,8 @@ struct ffr_renderer {
     double *dbv_pts = nullptr;
     unsigned long long *dhist = nullptr;
     void *dargs = nullptr;                // the walker kernel's arguments
+    void *dimg = nullptr;                 // image pixels (allocated on first use)
+    size_t dimg_bytes = 0;
     std::vector<unsigned char> hargs;     // their host copy, alive until the copy ran
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -289,6 +291,7 @@ void ffr_destroy(ffr_renderer *r)
     (void)hipFree(r->dbv_pts);
     (void)hipFree(r->dhist);
     (void)hipFree(r->dargs);
+    (void)hipFree(r->dimg);
     if (r->ev0)
         (void)hipEventDestroy(r->ev0);
     if (r->ev1)
@@ -854,6 +857,79 @@ int ffr_histogram_stats(ffr_renderer *r, uint64_t *sum, uint64_t *mn, uint64_t *
         *mn = (r->word_bytes == 4 && h[1] == ~0ull) ? 0xffffffffull : h[1];
     if (mx)
         *mx = h[2];
+    return FFR_OK;
+}
+
+size_t ffr_image_bytes(const ffr_renderer *r, int mode, int bits)
+{
+    if (!r || r->spec.dims != 2 || mode < FFR_IMG_MONO || mode > FFR_IMG_COLOR ||
+        (bits != 8 && bits != 16))
+        return 0;
+    const size_t per = mode == FFR_IMG_MONO ? 1 : (size_t)(bits / 8);
+    return (size_t)r->cells * (mode == FFR_IMG_COLOR ? 3 : 1) * per;
+}
+
+int ffr_render_image(ffr_renderer *r, int mode, double gamma, int bits, void *pixels,
+                     size_t bytes, ffr_image_info *info)
+{
+    if (!r)
+        return fail(FFR_E_ARG, "ffr_render_image: null renderer");
+    // the checks of ffr_img.cpp, in its order (:92-97, :130-134, :149-157,
+    // :236-237, :290-291); gamma and the log bound compare against eps of num_t
+    const bool f32 = r->word_bytes == 4;
+    if (f32 ? (float)gamma < 1e-10f : gamma < 1e-20)
+        return fail(FFR_E_ARG, "gamma too small");
+    if (bits != 8 && bits != 16)
+        return fail(FFR_E_ARG, "bits per channel must be 8 or 16");
+    if (r->spec.dims != 2)
+        return fail(FFR_E_UNSUPPORTED, "only 2D flames supported");
+    if (mode < FFR_IMG_MONO || mode > FFR_IMG_COLOR)
+        return fail(FFR_E_ARG,
+                    "must choose exactly 1 coloring flag (--monochrome, --grayscale, --color)");
+    const size_t need = ffr_image_bytes(r, mode, bits);
+    if (pixels && bytes != need)
+        return fail(FFR_E_SIZE, "ffr_render_image: expected " + std::to_string(need) +
+                                    " bytes of pixels");
+    HIP_TRY(hipSetDevice(r->device));
+    HIP_TRY((hipError_t)launch_hist_stats(r->dbuf, r->cells, (int)r->cell_words,
+                                          (int)r->word_bytes, r->dhist, r->stream));
+    unsigned long long h[3];
+    HIP_TRY(hipMemcpyAsync(h, r->dhist, sizeof h, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    const uint64_t mn = (f32 && h[1] == ~0ull) ? 0xffffffffull : h[1], mx = h[2];
+    // the log scaler's bounds (ffr_img.cpp:223-236), in num_t
+    const double smin = f32 ? (double)std::log(1.0f + (float)mn) : std::log(1.0 + (double)mn);
+    const double smax = f32 ? (double)std::log(1.0f + (float)mx) : std::log(1.0 + (double)mx);
+    if (info) {
+        info->hist_min = mn;
+        info->hist_max = mx;
+        info->scale_min = smin;
+        info->scale_max = smax;
+        info->width = (uint32_t)r->spec.size[0];
+        info->height = (uint32_t)r->spec.size[1];
+        info->channels = mode == FFR_IMG_COLOR ? 3u : 1u;
+        info->bytes_per_channel = mode == FFR_IMG_MONO ? 1u : (uint32_t)(bits / 8);
+    }
+    if (f32 ? (float)smax < 1e-10f : smax < 1e-20)
+        return fail(FFR_E_EMPTY, "histogram is (probably) empty");
+    if (mode == FFR_IMG_COLOR && r->spec.color_dims != 3)
+        return fail(FFR_E_UNSUPPORTED, "buffer must use 3 color dimensions");
+    if (!pixels)
+        return FFR_OK;
+    if (r->dimg_bytes < need) {
+        HIP_TRY(hipFree(r->dimg));
+        r->dimg = nullptr;
+        r->dimg_bytes = 0;
+        HIP_TRY(hipMalloc(&r->dimg, need));
+        r->dimg_bytes = need;
+    }
+    // num_t gp = 1.0 / arg_gamma (ffr_img.cpp:240): a double division, then num_t
+    const double gp = f32 ? (double)(float)(1.0 / (double)(float)gamma) : 1.0 / gamma;
+    HIP_TRY((hipError_t)launch_tonemap(r->dbuf, r->cells, (int)r->cell_words,
+                                       (int)r->word_bytes, mode, mx, gp,
+                                       mode == FFR_IMG_MONO ? 8 : bits, r->dimg, r->stream));
+    HIP_TRY(hipMemcpyAsync(pixels, r->dimg, need, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
     return FFR_OK;
 }
 

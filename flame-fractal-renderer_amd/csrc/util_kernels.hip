@@ -112,6 +112,43 @@ __global__ void l1_counts_kernel(const CA *a, int cwa, const CB *b, int cwb,
 
 __global__ void zero_double_kernel(double *p) { *p = 0.0; }
 
+// The image stage of ffr-img (ffr_img.cpp:199-309, image_renderer.hpp:
+// 150-192): one pixel per cell, rows in buffer order, in num_t arithmetic
+// (N): mono  count != 0 -> 1;  gray  pow(log(1 + count) / max, gp);
+// colour  that value times each colour sum / count.  A value v becomes the
+// pixel (P)(v * scale), scale = 2^bits (1 - 2^-mantissa), through a signed
+// 32-bit conversion as on the host (colour sums of an empty cell give
+// 0 / 0; such pixels are 0).  max = log(1 + largest count) is taken here, by
+// the same log as every cell's, so the brightest cell maps to exactly 1.
+template <typename CT, typename CR, typename N, typename P>
+__global__ void tonemap_kernel(const CT *buf, unsigned long long cells, int cw, int mode,
+                               N maxc, N gp, N scale, P *out)
+{
+    const N max = log((N)1 + maxc);
+    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         c < cells; c += (unsigned long long)gridDim.x * blockDim.x) {
+        const CT *cell = buf + c * (unsigned long long)cw;
+        const CT h = cell[0];
+        if (mode == 0) {
+            const N v = h != 0 ? (N)1.0 : (N)0.0;
+            out[c] = (P)(int32_t)(v * scale);
+            continue;
+        }
+        const N l = log((N)1 + (N)h) / max;
+        const N ll = pow(l, gp);
+        if (mode == 1) {
+            out[c] = (P)(int32_t)(ll * scale);
+            continue;
+        }
+        const N hn = (N)h;
+        for (int i = 0; i < 3; ++i) {
+            const CR col = reinterpret_cast<const CR *>(cell + 1)[i];
+            const N v = ll * ((N)col / hn);
+            out[c * 3 + i] = h != 0 ? (P)(int32_t)(v * scale) : (P)0;
+        }
+    }
+}
+
 __global__ void hist_stats_init_kernel(unsigned long long *out)
 {
     out[0] = 0;
@@ -198,6 +235,39 @@ int launch_l1_counts(const void *a, int cwa, int wba, const void *b, int cwb, in
     else
         hipLaunchKernelGGL((l1_counts_kernel<U4, U4>), dim3(grid), dim3(block), 0, s,
                            (const U4 *)a, cwa, (const U4 *)b, cwb, cells, inv_a, inv_b, out);
+    return (int)hipGetLastError();
+}
+
+int launch_tonemap(const void *buf, unsigned long long cells, int cw, int word_bytes, int mode,
+                   unsigned long long maxc, double gp, int bits, void *out, hipStream_t s)
+{
+    const int block = 256;
+    const int grid = std::min(grid_for(cells, block), 65536);
+    using U8 = unsigned long long;
+    using U4 = unsigned int;
+    if (word_bytes == 8) {
+        // the default build: num_t = double, 64-bit words
+        const double scale = (bits == 8 ? 256.0 : 65536.0) * (1.0 - 1.0 / 4503599627370496.0);
+        if (bits == 8)
+            hipLaunchKernelGGL((tonemap_kernel<U8, double, double, unsigned char>), dim3(grid),
+                               dim3(block), 0, s, (const U8 *)buf, cells, cw, mode, (double)maxc, gp,
+                               scale, (unsigned char *)out);
+        else
+            hipLaunchKernelGGL((tonemap_kernel<U8, double, double, unsigned short>), dim3(grid),
+                               dim3(block), 0, s, (const U8 *)buf, cells, cw, mode, (double)maxc, gp,
+                               scale, (unsigned short *)out);
+    } else {
+        // the float build: num_t = float, 32-bit words
+        const float scale = (bits == 8 ? 256.0f : 65536.0f) * (1.0f - 1.0f / 8388608.0f);
+        if (bits == 8)
+            hipLaunchKernelGGL((tonemap_kernel<U4, float, float, unsigned char>), dim3(grid),
+                               dim3(block), 0, s, (const U4 *)buf, cells, cw, mode, (float)maxc,
+                               (float)gp, scale, (unsigned char *)out);
+        else
+            hipLaunchKernelGGL((tonemap_kernel<U4, float, float, unsigned short>), dim3(grid),
+                               dim3(block), 0, s, (const U4 *)buf, cells, cw, mode, (float)maxc,
+                               (float)gp, scale, (unsigned short *)out);
+    }
     return (int)hipGetLastError();
 }
 

@@ -16,6 +16,11 @@ int launch_hist_stats(const void *buf, unsigned long long cells, int cw, int wor
 int launch_l1_counts(const void *a, int cwa, int wba, const void *b, int cwb, int wbb,
                      unsigned long long cells, double inv_a, double inv_b, double *out,
                      hipStream_t s);
+// the ffr-img image of a 2-D buffer: mode 0 mono, 1 gray, 2 colour (r = 3);
+// maxc = the largest count, gp = 1 / gamma in num_t; bits 8 or 16 (mono
+// always writes 8-bit pixels)
+int launch_tonemap(const void *buf, unsigned long long cells, int cw, int word_bytes, int mode,
+                   unsigned long long maxc, double gp, int bits, void *out, hipStream_t s);
 // zero a buffer with a kernel (usable inside stream capture)
 int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s);
 

@@ -111,8 +111,23 @@ class ServerStats(C.Structure):
     ]
 
 
+class ImageInfo(C.Structure):
+    _fields_ = [
+        ("hist_min", C.c_uint64),
+        ("hist_max", C.c_uint64),
+        ("scale_min", C.c_double),
+        ("scale_max", C.c_double),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("channels", C.c_uint32),
+        ("bytes_per_channel", C.c_uint32),
+    ]
+
+
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 FFR_E_TIMEOUT = 7
+FFR_E_EMPTY = 8
+IMG_MONO, IMG_GRAY, IMG_COLOR = 0, 1, 2
 
 
 # every symbol include/ffr.h declares
@@ -124,7 +139,7 @@ EXPORTS = [
     "ffr_add_buffer", "ffr_write_buffer", "ffr_clear_buffer",
     "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
     "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
-    "ffr_histogram_l1", "ffr_choose_precision",
+    "ffr_histogram_l1", "ffr_choose_precision", "ffr_image_bytes", "ffr_render_image",
     "ffr_batch_create", "ffr_batch_destroy", "ffr_batch_render", "ffr_batch_tune",
     "ffr_batch_set_graph", "ffr_batch_kinds", "ffr_batch_shape",
     "ffr_server_default_options", "ffr_server_create", "ffr_server_destroy",
@@ -186,6 +201,10 @@ def lib():
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                   C.c_size_t, C.POINTER(C.c_size_t)]
     L.ffr_histogram_l1.argtypes = [P, P, C.POINTER(C.c_double)]
+    L.ffr_image_bytes.restype = C.c_size_t
+    L.ffr_image_bytes.argtypes = [P, C.c_int, C.c_int]
+    L.ffr_render_image.argtypes = [P, C.c_int, C.c_double, C.c_int, P, C.c_size_t,
+                                   C.POINTER(ImageInfo)]
     L.ffr_choose_precision.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(Options), C.c_uint64,
                                        C.c_double, C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_double)]
@@ -353,6 +372,19 @@ class BufferRenderer:
         s, mn, mx = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(lib().ffr_histogram_stats(self._h, C.byref(s), C.byref(mn), C.byref(mx)))
         return s.value, mn.value, mx.value
+
+    def render_image(self, mode: int = IMG_GRAY, gamma: float = 1.0, bits: int = 8):
+        """The image ffr-img makes of this buffer (ffr_img.cpp:199-309), on the
+        device: an array (height, width) or (height, width, 3) of uint8 or
+        uint16 (mono is always uint8), and the bounds ffr-img prints."""
+        info = ImageInfo()
+        n = lib().ffr_image_bytes(self._h, mode, bits)
+        wide = mode != IMG_MONO and bits == 16
+        out = np.zeros(max(n // (2 if wide else 1), 1), dtype=np.uint16 if wide else np.uint8)
+        _check(lib().ffr_render_image(self._h, mode, float(gamma), bits,
+                                      out.ctypes.data if n else None, n, C.byref(info)))
+        shape = (info.height, info.width) + ((3,) if info.channels == 3 else ())
+        return out[: n // out.itemsize].reshape(shape), info
 
     # --- statistics getters (buffer_renderer.hpp:534-564) ---
     @property

@@ -34,7 +34,8 @@ enum {
     FFR_E_BADVALUE_LIMIT = 4,   /* render stopped: render() returned false     */
     FFR_E_SIZE = 5,             /* buffer size mismatch (addBuffer/readBuffer) */
     FFR_E_UNSUPPORTED = 6,      /* flame outside this build's limits           */
-    FFR_E_TIMEOUT = 7           /* a wait ran out of time                      */
+    FFR_E_TIMEOUT = 7,          /* a wait ran out of time                      */
+    FFR_E_EMPTY = 8             /* image of an empty histogram                 */
 };
 
 /* trajectory precision: num_t of the reference (src/types/types.hpp:29) */
@@ -128,6 +129,35 @@ int ffr_clear_buffer(ffr_renderer *r);
 
 /* histogramSum / histogramMin / histogramMax (:483-507), on the device */
 int ffr_histogram_stats(ffr_renderer *r, uint64_t *sum, uint64_t *min, uint64_t *max);
+
+/* The image stage of ffr-img (src/ffr_img.cpp:199-309 render_image, over
+ * ImageRenderer::renderGrayImage / renderColorImageRGB,
+ * src/renderers/image_renderer.hpp:150-192), on the device, for 2-D buffers.
+ * Pixels are row-major from buffer row 0, x fastest; 16-bit channels are in
+ * host order (a PNG writer swaps them).  Arithmetic is num_t of the buffer
+ * word: double for FFR_WORD64, float for FFR_WORD32.
+ *   FFR_IMG_MONO   one 8-bit channel (any bits): 255 where the count is not 0
+ *   FFR_IMG_GRAY   one channel: pow(log(1 + count) / log(1 + max count), 1/gamma)
+ *   FFR_IMG_COLOR  three channels: the gray value times colour sum / count
+ *                  (needs 3 colour dimensions; empty cells give 0)
+ * A value v becomes the pixel (pixel type)(v * 2^bits (1 - 2^-mantissa)),
+ * the reference's pix_scale (src/types/constants.hpp:77-91).  Errors carry
+ * the reference's texts: "gamma too small", "bits per channel must be 8 or
+ * 16", "histogram is (probably) empty" (FFR_E_EMPTY), "buffer must use 3
+ * color dimensions".  pixels may be NULL to fill only info (the bounds that
+ * ffr-img prints). */
+enum { FFR_IMG_MONO = 0, FFR_IMG_GRAY = 1, FFR_IMG_COLOR = 2 };
+typedef struct ffr_image_info {
+    uint64_t hist_min, hist_max;   /* count bounds ("histogram bounds")          */
+    double scale_min, scale_max;   /* log(1 + count) bounds ("scaler bounds")    */
+    uint32_t width, height;        /* cells in x and y                           */
+    uint32_t channels;             /* 1 or 3                                     */
+    uint32_t bytes_per_channel;    /* 1 or 2                                     */
+} ffr_image_info;
+/* bytes of the image for (mode, bits); 0 for a combination that is refused */
+size_t ffr_image_bytes(const ffr_renderer *r, int mode, int bits);
+int ffr_render_image(ffr_renderer *r, int mode, double gamma, int bits, void *pixels,
+                     size_t bytes, ffr_image_info *info);
 
 /* getXFormDistribution (:544-548): selections per xform id (file order) */
 int ffr_xform_distribution(const ffr_renderer *r, uint64_t *out, size_t n);

@@ -15,11 +15,16 @@
  *       reference's comparator (flame.hpp:291-293) on tied weights.
  *   the render loop (oracle_impl.h) pinned by closed forms, invariants and
  *       brute force; see the header of oracle_impl.h.
+ *   orc_tonemap                    pinned by closed forms of the pixel maps
+ *       (counts 0 / max / powers of e, gamma 1 and 2, colour = gray x mean
+ *       colour, the pix_scale rounding at exactly 1.0) and the reference's
+ *       error conditions (tests/test_oracle_image.py).
  */
 #define _GNU_SOURCE
 #include "oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -576,4 +581,106 @@ int orc_eval_variation(orc_flame *f, int32_t xform, int32_t var,
                        uint64_t walker, uint64_t *draw_index)
 {
     DISPATCH(evalvar, xform, var, in, out, seed, walker, draw_index)
+}
+
+/* ------------------------------------------------------------------ */
+/* ffr-img's image stage (src/ffr_img.cpp:199-309 render_image, over    */
+/* ImageRenderer::getValueBounds / renderGrayImage / renderColorImageRGB */
+/* src/renderers/image_renderer.hpp:110-192), cell by cell in num_t.     */
+/* ------------------------------------------------------------------ */
+
+/* (pix_t)(v * pix_scale): gcc on x86-64 converts through a signed 32-bit
+ * integer (cvttsd2si / cvttss2si) and keeps the low bits; NaN and values out
+ * of the int32 range give INT32_MIN there, whose low bits are 0 */
+static int32_t to_i32_d(double v)
+{
+    return (v != v || v >= 2147483648.0 || v < -2147483648.0) ? INT32_MIN : (int32_t)v;
+}
+static int32_t to_i32_f(float v)
+{
+    return (v != v || v >= 2147483648.0f || v < -2147483648.0f) ? INT32_MIN : (int32_t)v;
+}
+
+#define TONEMAP_BODY(N, CT, CR, LOG, POW, EPS, SCALE8, SCALE16, CONV)                   \
+    {                                                                                \
+        const CT *b = (const CT *)buf;                                               \
+        /* getValueBounds of the count and of log(1 + count) (:223-236) */          \
+        CT minh = b[0], maxh = b[0];                                                 \
+        N mn = LOG((N)1 + (N)b[0]), mx = mn;                                         \
+        for (uint64_t c = 0; c < cells; ++c) {                                       \
+            const CT h = b[c * (uint64_t)cw];                                        \
+            const N l = LOG((N)1 + (N)h);                                            \
+            if (h < minh) minh = h;                                                  \
+            if (h > maxh) maxh = h;                                                  \
+            if (l < mn) mn = l;                                                      \
+            if (l > mx) mx = l;                                                      \
+        }                                                                            \
+        hist[0] = (uint64_t)minh;                                                    \
+        hist[1] = (uint64_t)maxh;                                                    \
+        scaler[0] = (double)mn;                                                      \
+        scaler[1] = (double)mx;                                                      \
+        if (mx < EPS)                                                                \
+            return 1;                                                                \
+        if (!out)                                                                    \
+            return 0;                                                                \
+        /* num_t gp = 1.0 / arg_gamma: a double division, then num_t */             \
+        const N gp = (N)(1.0 / (double)(N)gamma);                                    \
+        const N scale = mode == 0 || bits == 8 ? SCALE8 : SCALE16;                   \
+        uint8_t *o8 = (uint8_t *)out;                                                \
+        uint16_t *o16 = (uint16_t *)out;                                             \
+        const int wide = mode != 0 && bits == 16;                                    \
+        for (uint64_t c = 0; c < cells; ++c) {                                       \
+            const CT *cell = b + c * (uint64_t)cw;                                   \
+            N v[3];                                                                  \
+            int nv = 1;                                                              \
+            if (mode == 0) {                                                         \
+                v[0] = cell[0] != 0 ? (N)1.0 : (N)0.0;                               \
+            } else {                                                                 \
+                const N l = LOG((N)1 + (N)cell[0]) / mx;                             \
+                const N ll = POW(l, gp);                                             \
+                if (mode == 1) {                                                     \
+                    v[0] = ll;                                                       \
+                } else {                                                             \
+                    const N h = (N)cell[0];                                          \
+                    const CR *col = (const CR *)(cell + 1);                          \
+                    for (int i = 0; i < 3; ++i)                                      \
+                        v[i] = ll * (col[i] / h);                                    \
+                    nv = 3;                                                          \
+                }                                                                    \
+            }                                                                        \
+            for (int i = 0; i < nv; ++i) {                                           \
+                const int32_t p = CONV(v[i] * scale);                                \
+                if (wide)                                                            \
+                    o16[c * (uint64_t)nv + (uint64_t)i] = (uint16_t)p;               \
+                else                                                                 \
+                    o8[c * (uint64_t)nv + (uint64_t)i] = (uint8_t)p;                 \
+            }                                                                        \
+        }                                                                            \
+        return 0;                                                                    \
+    }
+
+/* pix_scale (src/types/constants.hpp:77-91): 2^bits (1 - 2^-mantissa) */
+static int tonemap_f64(const void *buf, uint64_t cells, int32_t cw, int32_t mode,
+                       double gamma, int32_t bits, void *out, uint64_t hist[2],
+                       double scaler[2])
+TONEMAP_BODY(double, uint64_t, double, log, pow, 1e-20, 256.0 * (1.0 - 0x1p-52),
+             65536.0 * (1.0 - 0x1p-52), to_i32_d)
+
+static int tonemap_f32(const void *buf, uint64_t cells, int32_t cw, int32_t mode,
+                       double gamma, int32_t bits, void *out, uint64_t hist[2],
+                       double scaler[2])
+TONEMAP_BODY(float, uint32_t, float, logf, powf, 1e-10f, 256.0f * (1.0f - 0x1p-23f),
+             65536.0f * (1.0f - 0x1p-23f), to_i32_f)
+
+int orc_tonemap(const void *buf, uint64_t cells, int32_t cell_words, int32_t word_bytes,
+                int32_t mode, double gamma, int32_t bits, void *out, uint64_t hist[2],
+                double scaler[2])
+{
+    if (cells == 0 || (word_bytes != 4 && word_bytes != 8) || mode < 0 || mode > 2 ||
+        (bits != 8 && bits != 16) || (mode == 2 && cell_words != 4))
+        return 2;
+    if (word_bytes == 8)
+        return tonemap_f64(buf, cells, cell_words, mode, gamma, bits, out, hist, scaler);
+    return tonemap_f32(buf, cells, cell_words, mode, (double)(float)gamma, bits, out, hist,
+                       scaler);
 }

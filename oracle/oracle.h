@@ -131,6 +131,18 @@ int orc_render(orc_flame *f, void *buf, uint64_t seed, uint64_t walker_base,
                int32_t threads, uint64_t *xf_dist, orc_stats *stats,
                uint64_t *bv_xfs, double *bv_pts, uint64_t bv_cap);
 
+/*
+ * ffr-img's image of a 2-D buffer (src/ffr_img.cpp:199-309): mode 0 mono
+ * (8-bit, 255 where the count is not 0), 1 gray, 2 colour (cell_words 4);
+ * arithmetic in double (word_bytes 8) or float (4) with libm's log / pow.
+ * hist[] receives the count bounds, scaler[] the log(1 + count) bounds.
+ * out may be NULL (bounds only).  Returns 0, 1 for an empty histogram
+ * (nothing written), 2 for an argument the reference rejects.
+ */
+int orc_tonemap(const void *buf, uint64_t cells, int32_t cell_words, int32_t word_bytes,
+                int32_t mode, double gamma, int32_t bits, void *out, uint64_t hist[2],
+                double scaler[2]);
+
 /* the walker stream: word n of walker w under seed */
 uint64_t orc_stream_word(uint64_t seed, uint64_t walker, uint64_t n);
 /* raw Philox4x32-10 block */

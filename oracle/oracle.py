@@ -141,6 +141,11 @@ def lib():
             C.POINTER(C.c_uint64)]
         L.orc_sort_order.argtypes = [C.POINTER(C.c_double), C.c_int32,
                                      C.c_int32, C.POINTER(C.c_int32)]
+        L.orc_tonemap.restype = C.c_int
+        L.orc_tonemap.argtypes = [
+            C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, C.c_int32,
+            C.c_double, C.c_int32, C.c_void_p, C.POINTER(C.c_uint64),
+            C.POINTER(C.c_double)]
         L.orc_apply_xform.restype = C.c_int
         L.orc_apply_xform.argtypes = [
             C.c_void_p, C.c_int32, C.POINTER(C.c_double),
@@ -335,6 +340,26 @@ class OracleFlame:
         if rc != 0:
             raise IndexError("no such variation")
         return [out[i] for i in range(self.dims)], d.value
+
+
+def tonemap(buf: np.ndarray, width: int, height: int, cell_words: int, mode: int,
+            gamma: float = 1.0, bits: int = 8):
+    """ffr-img's image of a 2-D buffer (ffr_img.cpp:199-309); buf holds
+    width*height*cell_words words (uint64 or uint32).  Returns (status,
+    pixels or None, (hist_min, hist_max), (scale_min, scale_max)); status 0
+    ok, 1 empty histogram, 2 rejected argument."""
+    buf = np.ascontiguousarray(buf)
+    cells = width * height
+    assert buf.size == cells * cell_words
+    wide = mode != 0 and bits == 16
+    ch = 3 if mode == 2 else 1
+    out = np.zeros(cells * ch, dtype=np.uint16 if wide else np.uint8)
+    hist = (C.c_uint64 * 2)()
+    scaler = (C.c_double * 2)()
+    st = lib().orc_tonemap(buf.ctypes.data, cells, cell_words, buf.itemsize, mode,
+                           gamma, bits, out.ctypes.data, hist, scaler)
+    pix = out.reshape((height, width) + ((3,) if ch == 3 else ())) if st == 0 else None
+    return st, pix, (hist[0], hist[1]), (scaler[0], scaler[1])
 
 
 def stream_word(seed: int, walker: int, n: int) -> int:

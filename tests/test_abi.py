@@ -91,3 +91,33 @@ def test_cli_flame_echo_format(tmp_path):
     rc, _, err = _cli(["-f", str(f), "-o", "-"])
     # nlohmann-style dump: sorted keys, shortest doubles, ".0" on integral values
     assert 'flame: {"a":"x","b":[1e-20,100000.0,0.5,1e+21,-0.0,3],"dimensions":4}' in err
+
+
+def _img(args, stdin=None):
+    exe = os.path.join(PKG, "ffr-img")
+    assert os.path.exists(exe), "ffr-img is not built"
+    p = subprocess.run([exe] + args, input=stdin, capture_output=True, timeout=60)
+    return p.returncode, p.stdout, p.stderr.decode(errors="replace")
+
+
+def test_img_cli_help_and_option_errors(tmp_path):
+    """ffr-img's checks before any device work (ffr_img.cpp:80-135)"""
+    rc, out, err = _img([])
+    assert rc == 1 and "--grayscale" in err and out == b""
+    rc, _, err = _img(["-h"])
+    assert rc == 1 and "--gamma" in err
+    rc, _, err = _img(["-f", "x.json"])
+    assert rc == 1 and "'--output' is required" in err
+    rc, _, err = _img(["-f", "x.json", "-o", "-", "-y", "0"])
+    assert rc == 1 and "gamma too small" in err
+    rc, _, err = _img(["-f", "x.json", "-o", "-", "-y", "1e-21"])
+    assert rc == 1 and "gamma too small" in err
+    rc, _, err = _img(["-f", "x.json", "-o", "-", "-b", "12"])
+    assert rc == 1 and "bits per channel must be 8 or 16" in err
+    rc, _, err = _img(["-f", "x.json", "-o", "-", "-m", "-b16", "-y", "2.2"])
+    assert "--gamma 2.2\n--monochrome\n--bits 16\n--\n" in err
+    f = tmp_path / "f.json"
+    f.write_text(TF.tetrahedron())
+    rc, _, err = _img(["-f", str(f), "-o", "-", "-g", "-i", "a.buf"])
+    assert rc == 1 and "ERROR: only 2D flames supported" in err
+    assert "--input a.buf\n" in err and "flame: {" in err
