@@ -24,6 +24,11 @@ Also measured (sub-objects of `workloads`):
     to the render server at their arrival times; throughput and p50/p99 job
     latency, FP32 and FP64.  `p99_latency_s` is the FP32 serving p99.
 
+`image` is the ffr-img stage on the FP64 headline buffer (the reference's
+default build): the device tonemap kernel's time and bytes for colour at 8
+bits and gray at 16 bits, the wall time of the call that returns host
+pixels, and the CPU oracle's single-thread time for the same image.
+
 Prints one JSON line on stdout.  `roofline` is for the walker kernel:
 algorithmic bytes = plotted samples x the bytes each plot adds into its
 cell (32 B: an 8-byte count and three 8-byte colour sums; 8 B for the
@@ -103,8 +108,56 @@ def read(path):
 SETTLE_FP64 = 53
 
 
+# the image stage (ffr-img) on the rendered headline buffer: gamma 2.2,
+# colour at 8 bits (reads 32 B per cell, writes 3 B) and gray at 16 bits
+# (reads the 8-byte count of each 32-byte cell, writes 2 B)
+IMAGE_LEGS = (("color_8bit", 2, 8, 32), ("gray_16bit", 1, 16, 8))
+IMAGE_GAMMA = 2.2
+
+
+def image_leg(ffr, r, words, cpu):
+    """Device tonemap kernel time (HIP events on the renderer's stream, pixels
+    left in device memory), the host-pixel call's wall time (kernel, bounds
+    pass and the copy to host), and, unless cpu is False, the oracle's
+    single-thread image of the same buffer (the reference's render_image is
+    one thread) with the count of differing pixels."""
+    import numpy as np
+
+    out = {}
+    for name, mode, bits, read_b in IMAGE_LEGS:
+        n = r.image_bytes(mode, bits)
+        dev = ffr.DeviceMemory(n)
+        ms = []
+        for i in range(6):
+            info = r.render_image_device(dev.ptr, n, mode, IMAGE_GAMMA, bits)
+            if i:
+                ms.append(info.kernel_ms)
+        k = sum(ms) / len(ms)
+        alg = r.cells * read_b + n
+        t0 = time.perf_counter()
+        pix, info = r.render_image(mode, IMAGE_GAMMA, bits)
+        host_ms = 1e3 * (time.perf_counter() - t0)
+        res = {
+            "kernel_ms": k,
+            "algorithmic_bytes": alg,
+            "achieved_GBs": alg / (k * 1e-3) / 1e9,
+            "frac": alg / (k * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "host_pixels_ms": host_ms,
+        }
+        if cpu:
+            from oracle import oracle
+            w, h = int(info.width), int(info.height)
+            t0 = time.perf_counter()
+            st, ref, _, _ = oracle.tonemap(words, w, h, r.cell_words, mode, IMAGE_GAMMA, bits)
+            res["cpu_oracle_ms"] = 1e3 * (time.perf_counter() - t0)
+            res["pixels_differing_from_oracle"] = int(np.count_nonzero(pix != ref))
+        out[name] = res
+        dev.close()
+    return out
+
+
 def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, seed,
-            word=64, batch=0, settle=-1, keep=False):
+            word=64, batch=0, settle=-1, keep=False, image=False, cpu=True):
     r = ffr.BufferRenderer(text, precision=precision, word=word, seed=seed,
                            batch_size=batch, settle=settle)
     for _ in range(warmup):
@@ -145,6 +198,8 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
         t0 = time.perf_counter()
         words = r.read_buffer()
         out["read_buffer_ms"] = 1e3 * (time.perf_counter() - t0)
+        if image:
+            out["image"] = image_leg(ffr, r, words, cpu)
         hist = r.counts(words).astype("float64")
         del words
         hist /= max(hist.sum(), 1.0)
@@ -360,7 +415,8 @@ def main():
         log("c2 fp32", json.dumps(fp32))
         if not args.fp32_only:
             fp64, h64 = run_gpu(ffr, c2_text, precision="fp64", samples=C2_SAMPLES, steps=S,
-                                warmup=W, bytes_per_plot=32, seed=C2_SEED, keep=True)
+                                warmup=W, bytes_per_plot=32, seed=C2_SEED, keep=True,
+                                image=True, cpu=not args.no_cpu)
             l1_c2 = l1(h32, h64)
             del h32, h64
             log("c2 fp64", json.dumps(fp64), "L1", l1_c2)
@@ -491,6 +547,7 @@ def main():
             "kernel_ms": fp64["kernel_ms"],
             "roofline": fp64["roofline"],
         },
+        "image": None if fp64 is None else fp64.get("image"),
         "l1_fp32_vs_fp64": l1_c2,
         "p99_latency_s": workloads.get("serving_fp32", {}).get("latency_p99_s"),
         "workloads": workloads,

@@ -869,11 +869,16 @@ size_t ffr_image_bytes(const ffr_renderer *r, int mode, int bits)
     return (size_t)r->cells * (mode == FFR_IMG_COLOR ? 3 : 1) * per;
 }
 
-int ffr_render_image(ffr_renderer *r, int mode, double gamma, int bits, void *pixels,
-                     size_t bytes, ffr_image_info *info)
+namespace {
+
+// both image entry points: dst is host memory (to_host) or a device address
+int render_image(ffr_renderer *r, int mode, double gamma, int bits, void *dst, size_t bytes,
+                 ffr_image_info *info, bool to_host)
 {
     if (!r)
         return fail(FFR_E_ARG, "ffr_render_image: null renderer");
+    if (r->pending)
+        return fail(FFR_E_ARG, "ffr_render_image: a render is pending (call ffr_sync)");
     // the checks of ffr_img.cpp, in its order (:92-97, :130-134, :149-157,
     // :236-237, :290-291); gamma and the log bound compare against eps of num_t
     const bool f32 = r->word_bytes == 4;
@@ -887,7 +892,7 @@ int ffr_render_image(ffr_renderer *r, int mode, double gamma, int bits, void *pi
         return fail(FFR_E_ARG,
                     "must choose exactly 1 coloring flag (--monochrome, --grayscale, --color)");
     const size_t need = ffr_image_bytes(r, mode, bits);
-    if (pixels && bytes != need)
+    if (dst && (to_host ? bytes != need : bytes < need))
         return fail(FFR_E_SIZE, "ffr_render_image: expected " + std::to_string(need) +
                                     " bytes of pixels");
     HIP_TRY(hipSetDevice(r->device));
@@ -909,28 +914,54 @@ int ffr_render_image(ffr_renderer *r, int mode, double gamma, int bits, void *pi
         info->height = (uint32_t)r->spec.size[1];
         info->channels = mode == FFR_IMG_COLOR ? 3u : 1u;
         info->bytes_per_channel = mode == FFR_IMG_MONO ? 1u : (uint32_t)(bits / 8);
+        info->kernel_ms = 0.0;
     }
     if (f32 ? (float)smax < 1e-10f : smax < 1e-20)
         return fail(FFR_E_EMPTY, "histogram is (probably) empty");
     if (mode == FFR_IMG_COLOR && r->spec.color_dims != 3)
         return fail(FFR_E_UNSUPPORTED, "buffer must use 3 color dimensions");
-    if (!pixels)
+    if (!dst)
         return FFR_OK;
-    if (r->dimg_bytes < need) {
-        HIP_TRY(hipFree(r->dimg));
-        r->dimg = nullptr;
-        r->dimg_bytes = 0;
-        HIP_TRY(hipMalloc(&r->dimg, need));
-        r->dimg_bytes = need;
+    void *out = dst;
+    if (to_host) {
+        if (r->dimg_bytes < need) {
+            HIP_TRY(hipFree(r->dimg));
+            r->dimg = nullptr;
+            r->dimg_bytes = 0;
+            HIP_TRY(hipMalloc(&r->dimg, need));
+            r->dimg_bytes = need;
+        }
+        out = r->dimg;
     }
     // num_t gp = 1.0 / arg_gamma (ffr_img.cpp:240): a double division, then num_t
     const double gp = f32 ? (double)(float)(1.0 / (double)(float)gamma) : 1.0 / gamma;
+    HIP_TRY(hipEventRecord(r->ev0, r->stream));
     HIP_TRY((hipError_t)launch_tonemap(r->dbuf, r->cells, (int)r->cell_words,
                                        (int)r->word_bytes, mode, mx, gp,
-                                       mode == FFR_IMG_MONO ? 8 : bits, r->dimg, r->stream));
-    HIP_TRY(hipMemcpyAsync(pixels, r->dimg, need, hipMemcpyDeviceToHost, r->stream));
+                                       mode == FFR_IMG_MONO ? 8 : bits, out, r->stream));
+    HIP_TRY(hipEventRecord(r->ev1, r->stream));
+    if (to_host)
+        HIP_TRY(hipMemcpyAsync(dst, r->dimg, need, hipMemcpyDeviceToHost, r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    if (info)
+        info->kernel_ms = (double)ms;
     return FFR_OK;
+}
+
+} // namespace
+
+int ffr_render_image(ffr_renderer *r, int mode, double gamma, int bits, void *pixels,
+                     size_t bytes, ffr_image_info *info)
+{
+    return render_image(r, mode, gamma, bits, pixels, bytes, info, true);
+}
+
+int ffr_render_image_device(ffr_renderer *r, int mode, double gamma, int bits,
+                            void *device_pixels, size_t bytes, ffr_image_info *info)
+{
+    return render_image(r, mode, gamma, bits, device_pixels, bytes, info, false);
 }
 
 size_t ffr_xform_id_count(const ffr_renderer *r) { return r ? r->xf_dist.size() : 0; }

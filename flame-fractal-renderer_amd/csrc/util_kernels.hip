@@ -116,36 +116,62 @@ __global__ void zero_double_kernel(double *p) { *p = 0.0; }
 // 150-192): one pixel per cell, rows in buffer order, in num_t arithmetic
 // (N): mono  count != 0 -> 1;  gray  pow(log(1 + count) / max, gp);
 // colour  that value times each colour sum / count.  A value v becomes the
-// pixel (P)(v * scale), scale = 2^bits (1 - 2^-mantissa), through a signed
-// 32-bit conversion as on the host (colour sums of an empty cell give
-// 0 / 0; such pixels are 0).  max = log(1 + largest count) is taken here, by
-// the same log as every cell's, so the brightest cell maps to exactly 1.
+// pixel (P)(v * scale), scale = 2^bits (1 - 2^-mantissa).  max = log(1 +
+// largest count) is taken here, by the same log as every cell's, so the
+// brightest cell maps to exactly 1.
+
+// (pix_t)v as gcc compiles it for x86-64: a signed 32-bit conversion
+// (cvttsd2si / cvttss2si) of which the pixel keeps the low bits; NaN and
+// values outside the int32 range give INT32_MIN there, where the device's
+// own conversion would clamp
+template <typename N>
+__device__ inline int32_t host_i32(N v)
+{
+    return (v >= (N)-2147483648.0 && v < (N)2147483648.0) ? (int32_t)v : INT32_MIN;
+}
+
+// glibc's logf and powf (the float build's std::log / std::pow) evaluate
+// in double and round once, so they return the correctly rounded float in
+// all but a vanishing share of arguments; the device does the same
+__device__ inline double img_log(double x) { return log(x); }
+__device__ inline float img_log(float x) { return (float)log((double)x); }
+__device__ inline double img_pow(double x, double y) { return pow(x, y); }
+__device__ inline float img_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+
 template <typename CT, typename CR, typename N, typename P>
 __global__ void tonemap_kernel(const CT *buf, unsigned long long cells, int cw, int mode,
                                N maxc, N gp, N scale, P *out)
 {
-    const N max = log((N)1 + maxc);
+    const N max = img_log((N)1 + maxc);
+    // an empty cell is 0 in every mode while gp > 0: log(1) / max = 0 and
+    // pow(0, gp) = 0 in gray; in colour the sums over a zero count give
+    // NaN or infinity, whose conversion leaves 0
+    const bool empty_is_zero = gp > (N)0;
     for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
          c < cells; c += (unsigned long long)gridDim.x * blockDim.x) {
         const CT *cell = buf + c * (unsigned long long)cw;
         const CT h = cell[0];
         if (mode == 0) {
-            const N v = h != 0 ? (N)1.0 : (N)0.0;
-            out[c] = (P)(int32_t)(v * scale);
+            out[c] = (P)host_i32((h != 0 ? (N)1.0 : (N)0.0) * scale);
             continue;
         }
-        const N l = log((N)1 + (N)h) / max;
-        const N ll = pow(l, gp);
         if (mode == 1) {
-            out[c] = (P)(int32_t)(ll * scale);
+            N v = (N)0;
+            if (h != 0 || !empty_is_zero)
+                v = img_pow(img_log((N)1 + (N)h) / max, gp);
+            out[c] = (P)host_i32(v * scale);
             continue;
         }
-        const N hn = (N)h;
-        for (int i = 0; i < 3; ++i) {
-            const CR col = reinterpret_cast<const CR *>(cell + 1)[i];
-            const N v = ll * ((N)col / hn);
-            out[c * 3 + i] = h != 0 ? (P)(int32_t)(v * scale) : (P)0;
+        P px[3] = {0, 0, 0};
+        if (h != 0) {
+            const N ll = img_pow(img_log((N)1 + (N)h) / max, gp);
+            const N hn = (N)h;
+            const CR *col = reinterpret_cast<const CR *>(cell + 1);
+            for (int i = 0; i < 3; ++i)
+                px[i] = (P)host_i32(ll * ((N)col[i] / hn) * scale);
         }
+        for (int i = 0; i < 3; ++i)
+            out[c * 3 + i] = px[i];
     }
 }
 
@@ -241,8 +267,10 @@ int launch_l1_counts(const void *a, int cwa, int wba, const void *b, int cwb, in
 int launch_tonemap(const void *buf, unsigned long long cells, int cw, int word_bytes, int mode,
                    unsigned long long maxc, double gp, int bits, void *out, hipStream_t s)
 {
+    // 2048 workgroups of 256 (eight waves per SIMD); each lane takes about
+    // 16 cells of a 3840 x 2160 buffer, so the max above is taken rarely
     const int block = 256;
-    const int grid = std::min(grid_for(cells, block), 65536);
+    const int grid = std::min(grid_for(cells, block), 2048);
     using U8 = unsigned long long;
     using U4 = unsigned int;
     if (word_bytes == 8) {

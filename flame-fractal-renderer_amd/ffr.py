@@ -121,6 +121,7 @@ class ImageInfo(C.Structure):
         ("height", C.c_uint32),
         ("channels", C.c_uint32),
         ("bytes_per_channel", C.c_uint32),
+        ("kernel_ms", C.c_double),
     ]
 
 
@@ -140,6 +141,7 @@ EXPORTS = [
     "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
     "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
     "ffr_histogram_l1", "ffr_choose_precision", "ffr_image_bytes", "ffr_render_image",
+    "ffr_render_image_device",
     "ffr_batch_create", "ffr_batch_destroy", "ffr_batch_render", "ffr_batch_tune",
     "ffr_batch_set_graph", "ffr_batch_kinds", "ffr_batch_shape",
     "ffr_server_default_options", "ffr_server_create", "ffr_server_destroy",
@@ -205,6 +207,8 @@ def lib():
     L.ffr_image_bytes.argtypes = [P, C.c_int, C.c_int]
     L.ffr_render_image.argtypes = [P, C.c_int, C.c_double, C.c_int, P, C.c_size_t,
                                    C.POINTER(ImageInfo)]
+    L.ffr_render_image_device.argtypes = [P, C.c_int, C.c_double, C.c_int, P, C.c_size_t,
+                                          C.POINTER(ImageInfo)]
     L.ffr_choose_precision.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(Options), C.c_uint64,
                                        C.c_double, C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_double)]
@@ -258,6 +262,51 @@ def check_flame(text: str, precision: str = "fp64"):
     prec = FFR_FP32 if precision == "fp32" else FFR_FP64
     _check(L.ffr_check_flame(b, len(b), prec, ids, cw, cap, C.byref(n)))
     return [ids[i] for i in range(n.value)], [cw[i] for i in range(n.value)]
+
+
+class DeviceMemory:
+    """Device memory from the HIP runtime libffr_hip.so itself uses (one
+    runtime per process: a second HIP runtime, such as the one a PyTorch
+    wheel bundles, may not see the device once this one holds it)."""
+
+    _hip = None
+
+    @classmethod
+    def _runtime(cls):
+        if cls._hip is None:
+            lib()
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes: int, fill: int | None = None):
+        self.nbytes = nbytes
+        p = C.c_void_p()
+        if self._runtime().hipMalloc(C.byref(p), max(nbytes, 1)) != 0:
+            raise FfrError(3, f"hipMalloc of {nbytes} bytes failed")
+        self.ptr = p.value
+        if fill is not None and self._runtime().hipMemset(C.c_void_p(self.ptr), fill,
+                                                          nbytes) != 0:
+            raise FfrError(3, "hipMemset failed")
+
+    def to_host(self, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(self.nbytes, dtype=np.uint8)
+        if self._runtime().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                     self.nbytes, 2) != 0:  # device to host
+            raise FfrError(3, "hipMemcpy failed")
+        return out.view(dtype)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self._runtime().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
 
 
 class BufferRenderer:
@@ -373,6 +422,9 @@ class BufferRenderer:
         _check(lib().ffr_histogram_stats(self._h, C.byref(s), C.byref(mn), C.byref(mx)))
         return s.value, mn.value, mx.value
 
+    def image_bytes(self, mode: int = IMG_GRAY, bits: int = 8) -> int:
+        return int(lib().ffr_image_bytes(self._h, mode, bits))
+
     def render_image(self, mode: int = IMG_GRAY, gamma: float = 1.0, bits: int = 8):
         """The image ffr-img makes of this buffer (ffr_img.cpp:199-309), on the
         device: an array (height, width) or (height, width, 3) of uint8 or
@@ -385,6 +437,14 @@ class BufferRenderer:
                                       out.ctypes.data if n else None, n, C.byref(info)))
         shape = (info.height, info.width) + ((3,) if info.channels == 3 else ())
         return out[: n // out.itemsize].reshape(shape), info
+
+    def render_image_device(self, device_ptr: int, nbytes: int, mode: int = IMG_GRAY,
+                            gamma: float = 1.0, bits: int = 8):
+        """render_image into device memory (e.g. a torch tensor's data_ptr())"""
+        info = ImageInfo()
+        _check(lib().ffr_render_image_device(self._h, mode, float(gamma), bits,
+                                             C.c_void_p(device_ptr), nbytes, C.byref(info)))
+        return info
 
     # --- statistics getters (buffer_renderer.hpp:534-564) ---
     @property

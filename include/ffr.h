@@ -153,11 +153,17 @@ typedef struct ffr_image_info {
     uint32_t width, height;        /* cells in x and y                           */
     uint32_t channels;             /* 1 or 3                                     */
     uint32_t bytes_per_channel;    /* 1 or 2                                     */
+    double kernel_ms;              /* the pixel kernel, by HIP events (0 if none) */
 } ffr_image_info;
 /* bytes of the image for (mode, bits); 0 for a combination that is refused */
 size_t ffr_image_bytes(const ffr_renderer *r, int mode, int bits);
 int ffr_render_image(ffr_renderer *r, int mode, double gamma, int bits, void *pixels,
                      size_t bytes, ffr_image_info *info);
+/* the same image left in device memory: device_pixels is a device address
+ * of at least ffr_image_bytes bytes on the renderer's device; the call
+ * returns when the pixels are written */
+int ffr_render_image_device(ffr_renderer *r, int mode, double gamma, int bits,
+                            void *device_pixels, size_t bytes, ffr_image_info *info);
 
 /* getXFormDistribution (:544-548): selections per xform id (file order) */
 int ffr_xform_distribution(const ffr_renderer *r, uint64_t *out, size_t n);

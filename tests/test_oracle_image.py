@@ -186,3 +186,27 @@ def test_planted_mistake_is_caught(oracle_mod):
                 e = expect_pixel(exact_value(c, cmax, 1 / gamma), 8, 8, 1e-12)
                 wrong += e is not None and pix[y, x] != e
     assert wrong > 0.5 * np.count_nonzero((counts > 0) & (counts < cmax))
+
+
+def test_edge_values(oracle_mod):
+    """gamma = inf (gp = 0: pow(l, 0) = 1 everywhere, empty cells too),
+    gamma = NaN (passes the reference's check; pow(1, NaN) = 1, else NaN -> 0),
+    colour values outside [0, 1]: the pixel keeps the low bits of a signed
+    32-bit conversion, and values beyond the int32 range give 0"""
+    counts = np.zeros((H, W), dtype=np.int64)
+    counts[0, 0] = 4
+    counts[1, 1] = 2
+    colors = np.zeros((H, W, 3))
+    colors[0, 0] = [4 * 300.0, -4 * 0.5, 4 * 1e10]   # mean 300, -0.5, 1e10
+    colors[1, 1] = [1.0, 1.0, 1.0]
+    buf = word_buffer(counts, colors)
+    st, pix, _, _ = oracle_mod.tonemap(buf, W, H, 4, 1, float("inf"), 8)
+    assert st == 0 and (pix == 255).all()
+    st, pix, _, _ = oracle_mod.tonemap(buf, W, H, 4, 1, float("nan"), 8)
+    assert st == 0 and pix[0, 0] == 255 and int(pix.sum()) == 255
+    st, pix, _, _ = oracle_mod.tonemap(buf, W, H, 4, 2, 1.0, 8)
+    # 300 * 255.99.. = 76799.99.. -> 76799 -> low byte 255; -0.5 * 255.99.. =
+    # -127.99.. -> -127 -> 129; 2.56e12 is outside int32 -> 0
+    assert list(pix[0, 0]) == [76799 & 255, (-127) & 255, 0]
+    st, pix, _, _ = oracle_mod.tonemap(buf, W, H, 4, 2, 1.0, 16)
+    assert list(pix[0, 0]) == [(300 * 65536 - 1) & 0xFFFF, (-32767) & 0xFFFF, 0]
