@@ -75,6 +75,7 @@ struct ffr_renderer {
     double *dbv_pts = nullptr;
     unsigned long long *dhist = nullptr;
     void *dargs = nullptr;                // the walker kernel's arguments
+    unsigned *dhot = nullptr;             // hot-cell hints of the LDS cache
     void *dimg = nullptr;                 // image pixels (allocated on first use)
     size_t dimg_bytes = 0;
     std::vector<unsigned char> hargs;     // their host copy, alive until the copy ran
@@ -260,6 +261,10 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
     HIP_TRY_D(hipMalloc(&rp->dbv_pts, sizeof(double) * 3 * rp->opt.bad_value_cap));
     HIP_TRY_D(hipMalloc(&rp->dhist, sizeof(unsigned long long) * 3));
     HIP_TRY_D(hipMalloc(&rp->dargs, std::max(sizeof(WalkArgs<double>), sizeof(WalkArgs<float>))));
+    // hints start empty: a tag no geometry has
+    HIP_TRY_D(hipMalloc(&rp->dhot, sizeof(unsigned) * (1 + (size_t)kHotSlots)));
+    HIP_TRY_D(hipMemsetAsync(rp->dhot, 0xFF, sizeof(unsigned) * (1 + (size_t)kHotSlots),
+                             rp->stream));
     HIP_TRY_D((hipError_t)launch_fill_zero(rp->dbuf, bytes, rp->stream));
     HIP_TRY_D(hipStreamSynchronize(rp->stream));
 #undef HIP_TRY_D
@@ -292,6 +297,7 @@ void ffr_destroy(ffr_renderer *r)
     (void)hipFree(r->dhist);
     (void)hipFree(r->dargs);
     (void)hipFree(r->dimg);
+    (void)hipFree(r->dhot);
     if (r->ev0)
         (void)hipEventDestroy(r->ev0);
     if (r->ev1)
@@ -474,7 +480,7 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
 template <typename R>
 static void walk_args(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
                       uint64_t walker0, uint64_t n_walkers, uint64_t batch, uint64_t last,
-                      WalkArgs<R> &a)
+                      WalkArgs<R> &a, bool hints = false)
 {
     fill_args(r, d, a);
     a.walker_base = walker0;
@@ -486,6 +492,19 @@ static void walk_args(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan
     a.cache_log2 = p.log2s;
     a.cache_ways = p.ways;
     a.cache_direct = p.direct;
+    // hot-cell hints for hashed caches (a task keeps a slot as a hint when
+    // it counted at least an eighth of its samples per slot there, and at
+    // least hot_thresh); the tag names the cache geometry of the hints.
+    // Only for flames that few workgroups render at once: workgroups seeded
+    // alike miss the same cells, and their global atomics then queue on the
+    // same addresses (the headline flame on all 256 CUs ran 3x slower).
+    a.hot = nullptr;
+    a.hot_tag = 0;
+    a.hot_thresh = 2;
+    if (hints && p.cache && !p.direct && p.slots <= kHotSlots && r->dhot) {
+        a.hot = r->dhot;
+        a.hot_tag = 0x40000000u | ((unsigned)p.log2s << 8) | (unsigned)p.ways;
+    }
 }
 
 static int launch_walk_any(int mode, int dims, const WalkLaunch &L)

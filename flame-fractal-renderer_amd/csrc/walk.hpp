@@ -28,6 +28,10 @@ template <typename R> struct WalkArgs {
     unsigned long long *xf_count; // [nxf] selections by sorted position
     unsigned long long *bv_xfs;   // [bv_cap] sorted position of bad-value xforms
     double *bv_pts;               // [bv_cap][3]
+    // hot-cell hints of the flame's LDS cell cache: [tag, key of slot 0,
+    // key of slot 1, ...]; each task seeds its cache from them and leaves
+    // its own hot cells for the next (nullptr: no hints)
+    unsigned *hot;
     R lo[3], hi[3], mult_d[3];
     uint64_t mult_i[3];
     uint64_t cells;
@@ -46,6 +50,9 @@ template <typename R> struct WalkArgs {
     // [key u32 | count u32 | r colour words], probed cache_ways deep; when
     // cache_direct the cell index is the slot (the whole buffer fits)
     int cache_slots, cache_log2, cache_ways, cache_direct;
+    // hints apply when hot[0] == hot_tag (the same cache geometry); a slot
+    // is left as a hint when its task counted at least hot_thresh there
+    unsigned hot_tag, hot_thresh;
     // slot and final-variation descriptors live in the kernel arguments so
     // the kernel reads them with scalar loads
     int slot_type[kMaxSlots];
@@ -74,6 +81,9 @@ struct BatchTask {
     unsigned pad;
     uint64_t w_begin, w_end;
 };
+
+// hot-cell hint slots kept per renderer (caches with more slots take none)
+constexpr int kHotSlots = 1 << 15;
 
 // a walker-kernel launch: one flame (args, a device WalkArgs) or a batch
 // (descs, device WalkArgs[] indexed by the tasks' flame numbers)
