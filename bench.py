@@ -156,6 +156,24 @@ def image_leg(ffr, r, words, cpu):
     return out
 
 
+_atomic_peak = {}
+
+
+def atomic_roofline(ffr, atomics, kernel_ms, cells, cell_words, word_bytes):
+    """The walker kernel's global atomic adds per second (counted by the
+    kernel: misses of the LDS cell cache and its flushes) against the rate
+    the device sustains for cell updates scattered uniformly over a buffer
+    of the same shape (ffr_atomic_rate, 2^28 cell updates)."""
+    key = (cells, cell_words, word_bytes)
+    if key not in _atomic_peak:
+        _atomic_peak[key] = ffr.atomic_rate(cells, cell_words, word_bytes, 1 << 28)
+    peak = _atomic_peak[key] / 1e9
+    achieved = atomics / (kernel_ms * 1e-3) / 1e9
+    return {"achieved": achieved, "peak": peak, "unit": "G atomics/s",
+            "frac": achieved / peak if peak > 0 else None, "atomics_per_launch": atomics,
+            "peak_buffer": f"{cells} cells x {cell_words} words of {word_bytes} B"}
+
+
 def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, seed,
             word=64, batch=0, settle=-1, keep=False, image=False, cpu=True):
     r = ffr.BufferRenderer(text, precision=precision, word=word, seed=seed,
@@ -172,6 +190,7 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
         kms.append(r.kernel_ms)
     t1 = time.perf_counter()
     plotted = (r.samples_plotted - plot0) / steps
+    atomics = int(r.last_stats.atomics)
     wall = t1 - t0
     kavg = sum(kms) / len(kms)
     achieved = plotted * bytes_per_plot / (kavg * 1e-3) / 1e9
@@ -190,6 +209,8 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
         },
+        "atomic_roofline": atomic_roofline(ffr, atomics, kavg, r.cells, r.cell_words,
+                                           r.word_bytes),
     }
     hist = None
     if keep:
@@ -239,6 +260,8 @@ def run_batch(ffr, texts, samples, precision, keep):
         raise RuntimeError("a batched flame stopped at the bad-value limit")
     st = b.last_stats
     plotted = sum(r.samples_plotted for r in rs)
+    atomics = sum(int(r.last_stats.atomics) for r in rs)
+    big = max(rs, key=lambda r: r.cells)
     hists = None
     if keep:
         hists = []
@@ -264,6 +287,9 @@ def run_batch(ffr, texts, samples, precision, keep):
             "frac": plotted * 32 / (st.kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic": None,
         },
+        # priced at the largest flame's buffer shape (most of the samples)
+        "atomic_roofline": atomic_roofline(ffr, atomics, st.kernel_ms, big.cells,
+                                           big.cell_words, big.word_bytes),
     }
     b.close()
     for r in rs:
@@ -540,12 +566,14 @@ def main():
             "samples_per_walker": fp32["batch"],
         },
         "roofline": fp32["roofline"],
+        "atomic_roofline": fp32["atomic_roofline"],
         "kernel_ms": fp32["kernel_ms"],
         "fp64": None if fp64 is None else {
             "value": fp64["value"],
             "ms_per_step": fp64["ms_per_step"],
             "kernel_ms": fp64["kernel_ms"],
             "roofline": fp64["roofline"],
+            "atomic_roofline": fp64["atomic_roofline"],
         },
         "image": None if fp64 is None else fp64.get("image"),
         "l1_fp32_vs_fp64": l1_c2,

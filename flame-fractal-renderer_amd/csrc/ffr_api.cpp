@@ -88,7 +88,7 @@ struct ffr_renderer {
     std::vector<uint64_t> xf_dist; // by xform id
     double pmin[3], pmax[3];
     // last render
-    uint64_t last_bad = 0, last_walkers = 0, last_batch = 0;
+    uint64_t last_bad = 0, last_walkers = 0, last_batch = 0, last_atomics = 0;
     double last_ms = 0;
     std::vector<uint64_t> bv_ids;
     std::vector<double> bv_pts;
@@ -604,6 +604,7 @@ static int finish(ffr_renderer *r, ffr_stats *out)
             r->pmax[i] = std::max(r->pmax[i], st.pmax[i]);
         }
         r->last_bad = st.n_bad;
+        r->last_atomics = st.atomics;
         size_t nrec = (size_t)std::min<uint64_t>(st.n_bad, r->opt.bad_value_cap);
         r->bv_ids.assign(nrec, 0);
         r->bv_pts.assign(nrec * 3, 0.0);
@@ -628,6 +629,7 @@ static int finish(ffr_renderer *r, ffr_stats *out)
             out->pt_max[i] = i < r->spec.dims ? r->pmax[i] : 0.0;
         }
         out->kernel_ms = r->last_ms;
+        out->atomics = r->last_atomics;
     }
     if (r->last_bad > r->opt.bad_value_limit)
         return fail(FFR_E_BADVALUE_LIMIT, "render failure (flame may not be sufficiently contractive)");
@@ -877,6 +879,51 @@ int ffr_histogram_stats(ffr_renderer *r, uint64_t *sum, uint64_t *mn, uint64_t *
     if (mx)
         *mx = h[2];
     return FFR_OK;
+}
+
+int ffr_atomic_rate(int32_t device, uint64_t cells, uint32_t cell_words, uint32_t word_bytes,
+                    uint64_t updates, double *atomics_per_s)
+{
+    if (!atomics_per_s || cells == 0 || cells >= (1ull << 32) || cell_words == 0 ||
+        cell_words > 5 || (word_bytes != 4 && word_bytes != 8) || updates == 0)
+        return fail(FFR_E_ARG, "ffr_atomic_rate: cells in [1, 2^32), 1-5 words of 4 or 8 bytes, "
+                               "updates > 0");
+    HIP_TRY(hipSetDevice(device));
+    const size_t bytes = (size_t)cells * cell_words * word_bytes;
+    void *buf = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FFR_OK;
+    float ms = 0.f;
+    auto step = [&](hipError_t e, const char *what) {
+        if (rc == FFR_OK && e != hipSuccess)
+            rc = fail(FFR_E_DEVICE, std::string("ffr_atomic_rate: ") + what + ": " +
+                                        hipGetErrorString(e));
+        return rc == FFR_OK;
+    };
+    if (step(hipMalloc(&buf, bytes), "hipMalloc") &&
+        step(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream") &&
+        step(hipEventCreate(&e0), "event") && step(hipEventCreate(&e1), "event") &&
+        step((hipError_t)launch_fill_zero(buf, bytes, s), "clear") &&
+        step((hipError_t)launch_scatter_atomics(buf, cells, (int)cell_words, (int)word_bytes,
+                                                std::min<uint64_t>(updates, 1u << 24), s),
+             "warm-up") &&
+        step(hipEventRecord(e0, s), "record") &&
+        step((hipError_t)launch_scatter_atomics(buf, cells, (int)cell_words, (int)word_bytes,
+                                                updates, s),
+             "scatter") &&
+        step(hipEventRecord(e1, s), "record") && step(hipStreamSynchronize(s), "sync") &&
+        step(hipEventElapsedTime(&ms, e0, e1), "elapsed")) {
+        *atomics_per_s = ms > 0.f ? (double)updates * cell_words / ((double)ms * 1e-3) : 0.0;
+    }
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    if (s)
+        (void)hipStreamDestroy(s);
+    (void)hipFree(buf);
+    return rc;
 }
 
 size_t ffr_image_bytes(const ffr_renderer *r, int mode, int bits)

@@ -12,7 +12,7 @@ __global__ void reset_stats_kernel(DevStats *st, unsigned long long *xf_count, i
         st->s_iter = 0;
         st->s_plot = 0;
         st->n_bad = 0;
-        st->pad = 0;
+        st->atomics = 0;
         for (int i = 0; i < 3; ++i) {
             st->pmin[i] = (double)INFINITY;
             st->pmax[i] = -(double)INFINITY;
@@ -33,7 +33,7 @@ __global__ void reset_many_kernel(const ResetEntry *e, int n, unsigned *queues, 
             st->s_iter = 0;
             st->s_plot = 0;
             st->n_bad = 0;
-            st->pad = 0;
+            st->atomics = 0;
             for (int i = 0; i < 3; ++i) {
                 st->pmin[i] = (double)INFINITY;
                 st->pmax[i] = -(double)INFINITY;
@@ -175,6 +175,29 @@ __global__ void tonemap_kernel(const CT *buf, unsigned long long cells, int cw, 
     }
 }
 
+// Cell updates at cells chosen by a hash of the update index (a 64-bit
+// finaliser, uniform enough over any cell count), each one integer add on
+// the count word and a float add on every colour word, with the walker
+// kernel's instructions and scope.
+template <typename CT, typename CR>
+__global__ void scatter_atomics_kernel(CT *buf, unsigned long long cells, int cw,
+                                       unsigned long long updates)
+{
+    for (unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         u < updates; u += (unsigned long long)gridDim.x * blockDim.x) {
+        unsigned long long h = u * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        const unsigned long long c = __umul64hi(h, cells); // in [0, cells)
+        CT *cell = buf + c * (unsigned long long)cw;
+        __hip_atomic_fetch_add(cell, (CT)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        CR *col = reinterpret_cast<CR *>(cell + 1);
+        for (int i = 0; i < cw - 1; ++i)
+            unsafeAtomicAdd(&col[i], (CR)0.5);
+    }
+}
+
 __global__ void hist_stats_init_kernel(unsigned long long *out)
 {
     out[0] = 0;
@@ -296,6 +319,20 @@ int launch_tonemap(const void *buf, unsigned long long cells, int cw, int word_b
                                dim3(block), 0, s, (const U4 *)buf, cells, cw, mode, (float)maxc,
                                (float)gp, scale, (unsigned short *)out);
     }
+    return (int)hipGetLastError();
+}
+
+int launch_scatter_atomics(void *buf, unsigned long long cells, int cw, int word_bytes,
+                           unsigned long long updates, hipStream_t s)
+{
+    const int block = 256;
+    const int grid = std::min(grid_for(updates, block), 8192);
+    if (word_bytes == 8)
+        hipLaunchKernelGGL((scatter_atomics_kernel<unsigned long long, double>), dim3(grid),
+                           dim3(block), 0, s, (unsigned long long *)buf, cells, cw, updates);
+    else
+        hipLaunchKernelGGL((scatter_atomics_kernel<unsigned int, float>), dim3(grid), dim3(block),
+                           0, s, (unsigned int *)buf, cells, cw, updates);
     return (int)hipGetLastError();
 }
 

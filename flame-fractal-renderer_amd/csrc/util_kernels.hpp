@@ -21,6 +21,10 @@ int launch_l1_counts(const void *a, int cwa, int wba, const void *b, int cwb, in
 // always writes 8-bit pixels)
 int launch_tonemap(const void *buf, unsigned long long cells, int cw, int word_bytes, int mode,
                    unsigned long long maxc, double gp, int bits, void *out, hipStream_t s);
+// `updates` cell updates scattered uniformly over `cells` cells of cw words
+// (a count add and cw - 1 colour adds each, as the walker kernel's misses)
+int launch_scatter_atomics(void *buf, unsigned long long cells, int cw, int word_bytes,
+                           unsigned long long updates, hipStream_t s);
 // zero a buffer with a kernel (usable inside stream capture)
 int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s);
 

@@ -11,7 +11,7 @@ struct DevStats {
     unsigned long long s_iter;
     unsigned long long s_plot;
     unsigned long long n_bad;   // bad values seen; > limit stops new walkers
-    unsigned long long pad;
+    unsigned long long atomics; // global atomic adds on the buffer (misses, flushes)
     double pmin[3];
     double pmax[3];
 };

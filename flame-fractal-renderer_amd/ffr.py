@@ -58,6 +58,7 @@ class Stats(C.Structure):
         ("pt_min", C.c_double * 3),
         ("pt_max", C.c_double * 3),
         ("kernel_ms", C.c_double),
+        ("atomics", C.c_uint64),
     ]
 
 
@@ -140,7 +141,8 @@ EXPORTS = [
     "ffr_add_buffer", "ffr_write_buffer", "ffr_clear_buffer",
     "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
     "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
-    "ffr_histogram_l1", "ffr_choose_precision", "ffr_image_bytes", "ffr_render_image",
+    "ffr_histogram_l1", "ffr_choose_precision", "ffr_atomic_rate",
+    "ffr_image_bytes", "ffr_render_image",
     "ffr_render_image_device",
     "ffr_batch_create", "ffr_batch_destroy", "ffr_batch_render", "ffr_batch_tune",
     "ffr_batch_set_graph", "ffr_batch_kinds", "ffr_batch_shape",
@@ -203,6 +205,8 @@ def lib():
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                   C.c_size_t, C.POINTER(C.c_size_t)]
     L.ffr_histogram_l1.argtypes = [P, P, C.POINTER(C.c_double)]
+    L.ffr_atomic_rate.argtypes = [C.c_int32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64,
+                                  C.POINTER(C.c_double)]
     L.ffr_image_bytes.restype = C.c_size_t
     L.ffr_image_bytes.argtypes = [P, C.c_int, C.c_int]
     L.ffr_render_image.argtypes = [P, C.c_int, C.c_double, C.c_int, P, C.c_size_t,
@@ -262,6 +266,15 @@ def check_flame(text: str, precision: str = "fp64"):
     prec = FFR_FP32 if precision == "fp32" else FFR_FP64
     _check(L.ffr_check_flame(b, len(b), prec, ids, cw, cap, C.byref(n)))
     return [ids[i] for i in range(n.value)], [cw[i] for i in range(n.value)]
+
+
+def atomic_rate(cells: int, cell_words: int = 4, word_bytes: int = 8,
+                updates: int = 1 << 28, device: int = 0) -> float:
+    """Global atomic adds per second this device sustains with cell updates
+    scattered uniformly over `cells` cells (the atomic roofline)."""
+    out = C.c_double()
+    _check(lib().ffr_atomic_rate(device, cells, cell_words, word_bytes, updates, C.byref(out)))
+    return out.value
 
 
 class DeviceMemory:

@@ -72,6 +72,9 @@ typedef struct ffr_stats {
     double pt_min[3];         /* getPointExtremes (cumulative)                         */
     double pt_max[3];
     double kernel_ms;         /* device time of the last render's kernel (HIP events)  */
+    uint64_t atomics;         /* global atomic adds of the last render on the buffer:
+                                 one per word of a cell update that missed the LDS
+                                 cell cache or flushed it (no reference counterpart)  */
 } ffr_stats;
 
 /* defaults: FP64, WORD64, settle -1, device 0, seed 0, batch 0, limit 256,
@@ -129,6 +132,17 @@ int ffr_clear_buffer(ffr_renderer *r);
 
 /* histogramSum / histogramMin / histogramMax (:483-507), on the device */
 int ffr_histogram_stats(ffr_renderer *r, uint64_t *sum, uint64_t *min, uint64_t *max);
+
+/* The atomic roofline: the rate of global atomic adds this device sustains
+ * when cell updates are scattered uniformly over `cells` cells of
+ * `cell_words` words (word_bytes 8: a u64 add and f64 adds, as the walker
+ * kernel issues them for the default build's cells; 4: u32 and f32), with
+ * `updates` cell updates after one untimed pass.  *atomics_per_s receives
+ * word atomics per second (HIP events).  The buffer is allocated and freed
+ * here.  No counterpart in the reference: it prices the walker kernel's
+ * `atomics` against what the device can do. */
+int ffr_atomic_rate(int32_t device, uint64_t cells, uint32_t cell_words, uint32_t word_bytes,
+                    uint64_t updates, double *atomics_per_s);
 
 /* The image stage of ffr-img (src/ffr_img.cpp:199-309 render_image, over
  * ImageRenderer::renderGrayImage / renderColorImageRGB,
