@@ -83,14 +83,14 @@ C5_CHUNKS = 4
 C5_BUDGET = 200_000_000_000
 HBM_PEAK_GBS = 8000.0
 # walker kernel HBM bytes per launch on the headline workload (FP32):
-# FETCH_SIZE 852.5 KiB x 2.0 + WRITE_SIZE 18638580.2 KiB x 1.0 per dispatch.
+# FETCH_SIZE 894.69 KiB x 2.0 + WRITE_SIZE 18631702.5 KiB x 1.0 per dispatch.
 # The factors come from kernels of known traffic on the same buffer (the add
-# reads 2 x 265420800 B and reports 259231 KiB of FETCH_SIZE, the histogram
+# reads 2 x 265420800 B and reports 259224 KiB of FETCH_SIZE, the histogram
 # pass reads the buffer once and reports half of it; the clear and the add
 # each write 265420800 B and report exactly 259200 KiB of WRITE_SIZE).
 # Global atomics count 32 B each in WRITE_SIZE.  The raw values are in
 # profiles/walk_kernel_random_color_3840x2160_hbm.md.
-C2_FP32_TRAFFIC_BYTES = 852.5 * 1024 * 2.0 + 18638580.1875 * 1024 * 1.0
+C2_FP32_TRAFFIC_BYTES = 894.69 * 1024 * 2.0 + 18631702.5 * 1024 * 1.0
 
 
 def log(*a):
@@ -242,12 +242,12 @@ def read_batch_flames():
     return texts, cells
 
 
-def run_batch(ffr, texts, samples, precision, keep):
+def run_batch(ffr, texts, samples, precision, keep, seed0=C4_SEED):
     """One batched pass over all flames (after a tuning pass); returns the
     rates and, when keep, each flame's normalised histogram.  `precision` is
     one name for all flames or a list of one per flame."""
     precs = precision if isinstance(precision, list) else [precision] * len(texts)
-    rs = [ffr.BufferRenderer(t, precision=p, seed=C4_SEED + i, settle=SETTLE_FP64)
+    rs = [ffr.BufferRenderer(t, precision=p, seed=seed0 + i, settle=SETTLE_FP64)
           for i, (t, p) in enumerate(zip(texts, precs))]
     b = ffr.Batch(rs)
     t0 = time.perf_counter()
@@ -382,12 +382,14 @@ def calibration_pass(ffr, text):
     r.close()
 
 
-def cpu_rate(text, mode, seconds, seed, batch, settle=-1, fixed=None):
-    """Samples/s of the CPU oracle on all host threads: either `fixed`
-    samples, or about `seconds` of work sized from a short probe."""
+def cpu_rate(text, mode, seconds, seed, batch, settle=-1, fixed=None, cores=None):
+    """Samples/s of the CPU oracle on all host threads (or `cores`): either
+    `fixed` samples, or about `seconds` of work sized from a short probe."""
     from oracle import oracle
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    if cores is None:
+        cores = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                 else os.cpu_count())
     f = oracle.OracleFlame(text, mode, settle=settle)
     if fixed is None:
         n = 1 << 24
@@ -431,7 +433,7 @@ def main():
         calibration_pass(ffr, c2_text)
 
     fp32 = fp64 = None
-    l1_c2 = None
+    l1_c2 = floor_c2 = None
     if want("c2"):
         log(f"bench c2: {C2_SAMPLES} samples/step, warmup {W}, steps {S}")
         fp32, h32 = run_gpu(ffr, c2_text, precision="fp32", samples=C2_SAMPLES, steps=S,
@@ -444,8 +446,13 @@ def main():
                                 warmup=W, bytes_per_plot=32, seed=C2_SEED, keep=True,
                                 image=True, cpu=not args.no_cpu)
             l1_c2 = l1(h32, h64)
-            del h32, h64
-            log("c2 fp64", json.dumps(fp64), "L1", l1_c2)
+            del h32
+            # the sampling floor: FP64 against FP64 on other streams
+            _, h64b = run_gpu(ffr, c2_text, precision="fp64", samples=C2_SAMPLES, steps=1,
+                              warmup=0, bytes_per_plot=32, seed=C2_SEED + 0x100, keep=True)
+            floor_c2 = l1(h64, h64b)
+            del h64, h64b
+            log("c2 fp64", json.dumps(fp64), "L1", l1_c2, "floor", floor_c2)
 
     workloads = {}
     if want("c1") and not args.fp32_only:
@@ -459,6 +466,10 @@ def main():
         h64 = workloads["sierpinski_1024_fp64"].pop("hist")
         h32 = workloads["sierpinski_1024_fp32"].pop("hist")
         workloads["sierpinski_1024_fp32"]["l1_vs_fp64"] = l1(h32, h64)
+        _, h64b = run_gpu(ffr, text, precision="fp64", samples=C1_SAMPLES, steps=1, warmup=0,
+                          bytes_per_plot=8, seed=C1_SEED + 0x100, batch=65536, settle=20,
+                          keep=True)
+        workloads["sierpinski_1024_fp32"]["l1_fp64_vs_fp64_other_seed"] = l1(h64b, h64)
         if not args.no_cpu:
             rate, n, dt, cores = cpu_rate(text, oracle.F64_U64, 0, C1_SEED, 65536, settle=20,
                                           fixed=C1_SAMPLES)
@@ -490,6 +501,10 @@ def main():
         res32, h32 = run_batch(ffr, texts, samples, "fp32", keep=True)
         d32, res32["l1_vs_fp64"] = l1_summary(h32, h64)
         del h32
+        # the sampling floor per flame: FP64 against FP64 on other streams
+        _, h64b = run_batch(ffr, texts, samples, "fp64", keep=True, seed0=C4_SEED + 0x10000)
+        _, res32["l1_fp64_vs_fp64_other_seed"] = l1_summary(h64b, h64)
+        del h64b
         log("batch_256_fp32", json.dumps(res32))
         # the precision policy: a short probe per flame on the same streams;
         # FP64 where FP32 and FP64 drift apart
@@ -544,6 +559,10 @@ def main():
             "sample": f"{n} samples of the same flame in FP64 (walkers of 8192), "
                       f"{dt:.1f} s on {cores} host threads",
         }
+        rate1, n1, dt1, _ = cpu_rate(c2_text, oracle.F64_U64, args.cpu_seconds / 3, C2_SEED,
+                                     8192, cores=1)
+        cpu["one_thread"] = {"value": rate1, "cores": 1,
+                             "sample": f"{n1} samples, {dt1:.1f} s on one thread"}
         log("cpu", json.dumps(cpu))
     line = {
         "metric": "samples/sec and fraction of atomic/HBM roofline at FP32 vs FP64, 1 GPU; p99 latency",
@@ -577,6 +596,7 @@ def main():
         },
         "image": None if fp64 is None else fp64.get("image"),
         "l1_fp32_vs_fp64": l1_c2,
+        "l1_fp64_vs_fp64_other_seed": floor_c2,
         "p99_latency_s": workloads.get("serving_fp32", {}).get("latency_p99_s"),
         "workloads": workloads,
         "cpu_baseline": cpu,

@@ -64,11 +64,18 @@ __global__ void add_cells_kernel(CT *dst, const CT *src, unsigned long long word
     }
 }
 
+// reductions run 256-lane workgroups (four waves) and finish with one
+// atomic per workgroup
+constexpr int kReduceBlock = 256;
+constexpr int kReduceWaves = kReduceBlock / 64;
+constexpr int kReduceGrid = 512;
+
 template <typename CT>
 __global__ void hist_stats_kernel(const CT *buf, unsigned long long cells, int cw,
                                   unsigned long long *out /* sum, min, max */)
 {
     unsigned long long sum = 0, mn = ~0ull, mx = 0;
+#pragma unroll 4
     for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
          c < cells; c += (unsigned long long)gridDim.x * blockDim.x) {
         unsigned long long v = (unsigned long long)buf[c * (unsigned long long)cw];
@@ -83,7 +90,22 @@ __global__ void hist_stats_kernel(const CT *buf, unsigned long long cells, int c
         mn = a < mn ? a : mn;
         mx = b > mx ? b : mx;
     }
+    // one set of atomics per workgroup: atomics on one address serialise
+    // (about 80 M/s), so per-wave atomics over a 4K buffer took milliseconds
+    __shared__ unsigned long long ps[kReduceWaves], pn[kReduceWaves], px[kReduceWaves];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        ps[w] = sum;
+        pn[w] = mn;
+        px[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+            sum += ps[i];
+            mn = pn[i] < mn ? pn[i] : mn;
+            mx = px[i] > mx ? px[i] : mx;
+        }
         atomicAdd(&out[0], sum);
         atomicMin(&out[1], mn);
         atomicMax(&out[2], mx);
@@ -98,6 +120,7 @@ __global__ void l1_counts_kernel(const CA *a, int cwa, const CB *b, int cwb,
                                  double *out)
 {
     double s = 0.0;
+#pragma unroll 4
     for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
          c < cells; c += (unsigned long long)gridDim.x * blockDim.x) {
         const double d = (double)a[c * (unsigned long long)cwa] * inv_a -
@@ -106,8 +129,15 @@ __global__ void l1_counts_kernel(const CA *a, int cwa, const CB *b, int cwb,
     }
     for (int off = 32; off > 0; off >>= 1)
         s += __shfl_xor(s, off, 64);
+    __shared__ double part[kReduceWaves];
     if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+            s += part[i];
         atomicAdd(out, s);
+    }
 }
 
 __global__ void zero_double_kernel(double *p) { *p = 0.0; }
@@ -252,8 +282,8 @@ int launch_hist_stats(const void *buf, unsigned long long cells, int cw, int wor
                       unsigned long long *out, hipStream_t s)
 {
     hipLaunchKernelGGL(hist_stats_init_kernel, dim3(1), dim3(1), 0, s, out);
-    const int block = 256;
-    const int grid = grid_for(cells, block);
+    const int block = kReduceBlock;
+    const int grid = std::min(grid_for(cells, block), kReduceGrid);
     if (word_bytes == 8)
         hipLaunchKernelGGL((hist_stats_kernel<unsigned long long>), dim3(grid), dim3(block), 0,
                            s, (const unsigned long long *)buf, cells, cw, out);
@@ -267,8 +297,8 @@ int launch_l1_counts(const void *a, int cwa, int wba, const void *b, int cwb, in
                      unsigned long long cells, double inv_a, double inv_b, double *out,
                      hipStream_t s)
 {
-    const int block = 256;
-    const int grid = std::min(grid_for(cells, block), 2048);
+    const int block = kReduceBlock;
+    const int grid = std::min(grid_for(cells, block), kReduceGrid);
     hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(1), 0, s, out);
     using U8 = unsigned long long;
     using U4 = unsigned int;
