@@ -159,7 +159,7 @@ def image_leg(ffr, r, words, cpu):
 _atomic_peak = {}
 
 
-def atomic_roofline(ffr, atomics, kernel_ms, cells, cell_words, word_bytes):
+def atomic_roofline(ffr, atomics, kernel_ms, cells, cell_words, word_bytes, flush_atomics=None):
     """The walker kernel's global atomic adds per second (counted by the
     kernel: misses of the LDS cell cache and its flushes) against the rate
     the device sustains for cell updates scattered uniformly over a buffer
@@ -169,9 +169,15 @@ def atomic_roofline(ffr, atomics, kernel_ms, cells, cell_words, word_bytes):
         _atomic_peak[key] = ffr.atomic_rate(cells, cell_words, word_bytes, 1 << 28)
     peak = _atomic_peak[key] / 1e9
     achieved = atomics / (kernel_ms * 1e-3) / 1e9
-    return {"achieved": achieved, "peak": peak, "unit": "G atomics/s",
-            "frac": achieved / peak if peak > 0 else None, "atomics_per_launch": atomics,
-            "peak_buffer": f"{cells} cells x {cell_words} words of {word_bytes} B"}
+    out = {"achieved": achieved, "peak": peak, "unit": "G atomics/s",
+           "frac": achieved / peak if peak > 0 else None, "atomics_per_launch": atomics,
+           "peak_buffer": f"{cells} cells x {cell_words} words of {word_bytes} B"}
+    if flush_atomics is not None:
+        # the LDS cache's flushes, reported apart: one 32-B atomic per word
+        # of every occupied slot at the end of each task
+        out["flush_atomics_per_launch"] = flush_atomics
+        out["flush_bytes_per_launch"] = flush_atomics * 32
+    return out
 
 
 def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, seed,
@@ -191,6 +197,7 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
     t1 = time.perf_counter()
     plotted = (r.samples_plotted - plot0) / steps
     atomics = int(r.last_stats.atomics)
+    flush_atomics = int(r.last_stats.flush_atomics)
     wall = t1 - t0
     kavg = sum(kms) / len(kms)
     achieved = plotted * bytes_per_plot / (kavg * 1e-3) / 1e9
@@ -210,7 +217,7 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
             "traffic": None,
         },
         "atomic_roofline": atomic_roofline(ffr, atomics, kavg, r.cells, r.cell_words,
-                                           r.word_bytes),
+                                           r.word_bytes, flush_atomics),
     }
     hist = None
     if keep:
@@ -219,6 +226,15 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
         t0 = time.perf_counter()
         words = r.read_buffer()
         out["read_buffer_ms"] = 1e3 * (time.perf_counter() - t0)
+
+        class _Sink:
+            def write(self, b):
+                return len(b)
+        # the streamed read: pinned chunks, the copy of one overlapping the
+        # caller's write of the last (here a sink that keeps nothing)
+        t0 = time.perf_counter()
+        r.write_buffer_stream(_Sink())
+        out["read_buffer_stream_ms"] = 1e3 * (time.perf_counter() - t0)
         if image:
             out["image"] = image_leg(ffr, r, words, cpu)
         hist = r.counts(words).astype("float64")
@@ -261,6 +277,7 @@ def run_batch(ffr, texts, samples, precision, keep, seed0=C4_SEED):
     st = b.last_stats
     plotted = sum(r.samples_plotted for r in rs)
     atomics = sum(int(r.last_stats.atomics) for r in rs)
+    flush_atomics = sum(int(r.last_stats.flush_atomics) for r in rs)
     big = max(rs, key=lambda r: r.cells)
     hists = None
     if keep:
@@ -289,7 +306,7 @@ def run_batch(ffr, texts, samples, precision, keep, seed0=C4_SEED):
         },
         # priced at the largest flame's buffer shape (most of the samples)
         "atomic_roofline": atomic_roofline(ffr, atomics, st.kernel_ms, big.cells,
-                                           big.cell_words, big.word_bytes),
+                                           big.cell_words, big.word_bytes, flush_atomics),
     }
     b.close()
     for r in rs:

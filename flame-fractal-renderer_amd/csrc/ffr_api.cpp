@@ -76,6 +76,11 @@ struct ffr_renderer {
     unsigned long long *dhist = nullptr;
     void *dargs = nullptr;                // the walker kernel's arguments
     unsigned *dhot = nullptr;             // hot-cell hints of the LDS cache
+    // streamed buffer I/O: two pinned host chunks, two device chunks
+    void *pin[2] = {nullptr, nullptr};
+    void *dchunk[2] = {nullptr, nullptr};
+    hipEvent_t io_ev[2] = {nullptr, nullptr};
+    size_t io_chunk = 0;
     void *dimg = nullptr;                 // image pixels (allocated on first use)
     size_t dimg_bytes = 0;
     std::vector<unsigned char> hargs;     // their host copy, alive until the copy ran
@@ -88,7 +93,8 @@ struct ffr_renderer {
     std::vector<uint64_t> xf_dist; // by xform id
     double pmin[3], pmax[3];
     // last render
-    uint64_t last_bad = 0, last_walkers = 0, last_batch = 0, last_atomics = 0;
+    uint64_t last_bad = 0, last_walkers = 0, last_batch = 0, last_atomics = 0,
+             last_flush_atomics = 0;
     double last_ms = 0;
     std::vector<uint64_t> bv_ids;
     std::vector<double> bv_pts;
@@ -298,6 +304,13 @@ void ffr_destroy(ffr_renderer *r)
     (void)hipFree(r->dargs);
     (void)hipFree(r->dimg);
     (void)hipFree(r->dhot);
+    for (int k = 0; k < 2; ++k) {
+        if (r->pin[k])
+            (void)hipHostFree(r->pin[k]);
+        (void)hipFree(r->dchunk[k]);
+        if (r->io_ev[k])
+            (void)hipEventDestroy(r->io_ev[k]);
+    }
     if (r->ev0)
         (void)hipEventDestroy(r->ev0);
     if (r->ev1)
@@ -605,6 +618,7 @@ static int finish(ffr_renderer *r, ffr_stats *out)
         }
         r->last_bad = st.n_bad;
         r->last_atomics = st.atomics;
+        r->last_flush_atomics = st.flush_atomics;
         size_t nrec = (size_t)std::min<uint64_t>(st.n_bad, r->opt.bad_value_cap);
         r->bv_ids.assign(nrec, 0);
         r->bv_pts.assign(nrec * 3, 0.0);
@@ -630,6 +644,7 @@ static int finish(ffr_renderer *r, ffr_stats *out)
         }
         out->kernel_ms = r->last_ms;
         out->atomics = r->last_atomics;
+        out->flush_atomics = r->last_flush_atomics;
     }
     if (r->last_bad > r->opt.bad_value_limit)
         return fail(FFR_E_BADVALUE_LIMIT, "render failure (flame may not be sufficiently contractive)");
@@ -743,6 +758,117 @@ int ffr_add_buffer(ffr_renderer *r, const void *src, size_t bytes)
     }
     (void)hipFree(tmp);
     return rc;
+}
+
+} // extern "C"
+
+namespace {
+
+// the streamed I/O chunks: 32 MiB of whole cells (or the whole buffer),
+// allocated on first use and kept until ffr_destroy
+int io_init(ffr_renderer *r)
+{
+    if (r->io_chunk)
+        return FFR_OK;
+    const size_t cell_bytes = (size_t)r->cell_words * r->word_bytes;
+    const size_t chunk =
+        std::min(ffr_buffer_bytes(r), std::max<size_t>(1, (size_t)(32u << 20) / cell_bytes) *
+                                          cell_bytes);
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(hipHostMalloc(&r->pin[k], chunk, hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&r->dchunk[k], chunk));
+        HIP_TRY(hipEventCreateWithFlags(&r->io_ev[k], hipEventDisableTiming));
+    }
+    r->io_chunk = chunk;
+    return FFR_OK;
+}
+
+// fill dst from the callback until n bytes or the end
+size_t read_full(ffr_read_fn read, void *ctx, void *dst, size_t n)
+{
+    size_t got = 0;
+    while (got < n) {
+        const size_t k = read(ctx, (char *)dst + got, n - got);
+        if (k == 0)
+            break;
+        got += std::min(k, n - got);
+    }
+    return got;
+}
+
+} // namespace
+
+extern "C" {
+
+int ffr_add_buffer_stream(ffr_renderer *r, ffr_read_fn read, void *ctx)
+{
+    if (!r || !read)
+        return fail(FFR_E_ARG, "ffr_add_buffer_stream: null argument");
+    if (r->pending)
+        return fail(FFR_E_ARG, "ffr_add_buffer_stream: a render is pending (call ffr_sync)");
+    HIP_TRY(hipSetDevice(r->device));
+    const int rc0 = io_init(r);
+    if (rc0 != FFR_OK)
+        return rc0;
+    const size_t bytes = ffr_buffer_bytes(r), chunk = r->io_chunk;
+    // chunk i goes through pinned buffer i % 2: before it is refilled, the
+    // add of chunk i - 2 (which read it through its device chunk) is waited for
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += chunk, ++i) {
+        const int k = (int)(i & 1);
+        const size_t n = std::min(chunk, bytes - off);
+        if (i >= 2)
+            HIP_TRY(hipEventSynchronize(r->io_ev[k]));
+        if (read_full(read, ctx, r->pin[k], n) != n) {
+            HIP_TRY(hipStreamSynchronize(r->stream));
+            return fail(FFR_E_IO, "error reading file");
+        }
+        HIP_TRY(hipMemcpyAsync(r->dchunk[k], r->pin[k], n, hipMemcpyHostToDevice, r->stream));
+        HIP_TRY((hipError_t)launch_add_cells((char *)r->dbuf + off, r->dchunk[k],
+                                             n / r->word_bytes, (int)r->cell_words,
+                                             (int)r->word_bytes, r->stream));
+        HIP_TRY(hipEventRecord(r->io_ev[k], r->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return FFR_OK;
+}
+
+int ffr_read_buffer_stream(ffr_renderer *r, ffr_write_fn write, void *ctx)
+{
+    if (!r || !write)
+        return fail(FFR_E_ARG, "ffr_read_buffer_stream: null argument");
+    if (r->pending)
+        return fail(FFR_E_ARG, "ffr_read_buffer_stream: a render is pending (call ffr_sync)");
+    HIP_TRY(hipSetDevice(r->device));
+    const int rc0 = io_init(r);
+    if (rc0 != FFR_OK)
+        return rc0;
+    const size_t bytes = ffr_buffer_bytes(r), chunk = r->io_chunk;
+    const size_t nchunks = (bytes + chunk - 1) / chunk;
+    auto issue = [&](size_t i) {
+        const int k = (int)(i & 1);
+        const size_t off = i * chunk, n = std::min(chunk, bytes - off);
+        hipError_t e = hipMemcpyAsync(r->pin[k], (const char *)r->dbuf + off, n,
+                                      hipMemcpyDeviceToHost, r->stream);
+        if (e == hipSuccess)
+            e = hipEventRecord(r->io_ev[k], r->stream);
+        return e;
+    };
+    // the copy of chunk i + 1 runs while chunk i is written out; pinned
+    // buffer (i + 1) % 2 held chunk i - 1, which is written out already
+    HIP_TRY(issue(0));
+    for (size_t i = 0; i < nchunks; ++i) {
+        const int k = (int)(i & 1);
+        if (i + 1 < nchunks)
+            HIP_TRY(issue(i + 1));
+        HIP_TRY(hipEventSynchronize(r->io_ev[k]));
+        const size_t n = std::min(chunk, bytes - i * chunk);
+        if (write(ctx, r->pin[k], n) != n) {
+            HIP_TRY(hipStreamSynchronize(r->stream));
+            return fail(FFR_E_IO, "error writing output");
+        }
+    }
+    return FFR_OK;
 }
 
 int ffr_check_flame(const char *flame_json, size_t len, uint32_t precision, int32_t *ids,

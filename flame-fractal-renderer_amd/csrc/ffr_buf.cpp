@@ -198,6 +198,21 @@ bool read_all(std::istream &is, std::string &out)
     return !is.bad();
 }
 
+// stream callbacks of ffr_add_buffer_stream / ffr_read_buffer_stream
+size_t istream_read(void *ctx, void *dst, size_t n)
+{
+    auto *is = static_cast<std::istream *>(ctx);
+    is->read(static_cast<char *>(dst), (std::streamsize)n);
+    return (size_t)is->gcount();
+}
+
+size_t ostream_write(void *ctx, const void *src, size_t n)
+{
+    auto *os = static_cast<std::ostream *>(ctx);
+    os->write(static_cast<const char *>(src), (std::streamsize)n);
+    return os->good() ? n : 0;
+}
+
 double now_sec()
 {
     struct timespec t;
@@ -316,32 +331,21 @@ int main(int argc, char **argv)
     std::cerr << std::endl;
     std::cerr << "color: " << ffr_color_dims(r) << " dimensions" << std::endl;
 
-    // add the input buffers (ffr_buf.cpp:167-183)
-    std::vector<char> in((size_t)bytes);
+    // add the input buffers (ffr_buf.cpp:167-183), streamed to the device
     for (auto &s : a.inputs) {
         std::cerr << "adding input file " << s << std::endl;
-        bool ok;
-        if (s == "-") {
-            std::cin.read(in.data(), (std::streamsize)bytes);
-            ok = (size_t)std::cin.gcount() == bytes;
-        } else {
-            std::ifstream f(s, std::ios::in | std::ios::binary);
-            f.read(in.data(), (std::streamsize)bytes);
-            ok = f.good() || (size_t)f.gcount() == bytes;
-            ok = ok && (size_t)f.gcount() == bytes;
+        std::ifstream f;
+        std::istream *is = &std::cin;
+        if (s != "-") {
+            f.open(s, std::ios::in | std::ios::binary);
+            is = &f;
         }
-        if (!ok) {
-            std::cerr << "ERROR: error reading file" << std::endl;
-            ffr_destroy(r);
-            return 1;
-        }
-        if (ffr_add_buffer(r, in.data(), bytes) != FFR_OK) {
+        if (ffr_add_buffer_stream(r, istream_read, is) != FFR_OK) {
             std::cerr << "ERROR: " << ffr_last_error() << std::endl;
             ffr_destroy(r);
             return 1;
         }
     }
-    std::vector<char>().swap(in);
 
     // render (ffr_buf.cpp:185-257)
     if (a.samples > 0) {
@@ -402,26 +406,22 @@ int main(int argc, char **argv)
         std::cerr << std::endl;
     }
 
-    // write the buffer (ffr_buf.cpp:259-272)
+    // write the buffer (ffr_buf.cpp:259-272), streamed from the device
     std::cerr << "writing output" << std::endl;
-    std::vector<char> out((size_t)bytes);
-    if (ffr_read_buffer(r, out.data(), bytes) != FFR_OK) {
-        std::cerr << "ERROR: " << ffr_last_error() << std::endl;
-        ffr_destroy(r);
-        return 1;
+    std::ofstream f;
+    std::ostream *os = &std::cout;
+    if (a.output != "-") {
+        f.open(a.output, std::ios::out | std::ios::binary);
+        os = &f;
     }
+    int wrc = os->good() ? ffr_read_buffer_stream(r, ostream_write, os) : FFR_E_IO;
     ffr_destroy(r);
-    bool ok;
-    if (a.output == "-") {
-        std::cout.write(out.data(), (std::streamsize)bytes);
-        std::cout.flush();
-        ok = std::cout.good();
-    } else {
-        std::ofstream f(a.output, std::ios::out | std::ios::binary);
-        f.write(out.data(), (std::streamsize)bytes);
-        ok = f.good();
+    if (wrc == FFR_OK) {
+        os->flush();
+        if (!os->good())
+            wrc = FFR_E_IO;
     }
-    if (!ok) {
+    if (wrc != FFR_OK) {
         std::cerr << "ERROR: error writing output" << std::endl;
         return 1;
     }

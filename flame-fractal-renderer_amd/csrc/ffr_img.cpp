@@ -178,6 +178,13 @@ int parse_args(int argc, char **argv, Args &a)
     return 0;
 }
 
+size_t istream_read(void *ctx, void *dst, size_t n)
+{
+    auto *is = static_cast<std::istream *>(ctx);
+    is->read(static_cast<char *>(dst), (std::streamsize)n);
+    return (size_t)is->gcount();
+}
+
 bool read_all(std::istream &is, std::string &out)
 {
     out.assign(std::istreambuf_iterator<char>(is), std::istreambuf_iterator<char>());
@@ -399,31 +406,21 @@ int main(int argc, char **argv)
         return 1;
     }
 
-    // add the input buffers (ffr_img.cpp:166-183)
-    std::vector<char> in(bytes);
+    // add the input buffers (ffr_img.cpp:166-183), streamed to the device
     for (auto &s : a.inputs) {
         std::cerr << "adding input file " << s << std::endl;
-        bool ok;
-        if (s == "-") {
-            std::cin.read(in.data(), (std::streamsize)bytes);
-            ok = (size_t)std::cin.gcount() == bytes;
-        } else {
-            std::ifstream f(s, std::ios::in | std::ios::binary);
-            f.read(in.data(), (std::streamsize)bytes);
-            ok = (size_t)f.gcount() == bytes;
+        std::ifstream f;
+        std::istream *is = &std::cin;
+        if (s != "-") {
+            f.open(s, std::ios::in | std::ios::binary);
+            is = &f;
         }
-        if (!ok) {
-            std::cerr << "ERROR: error reading file" << std::endl;
-            ffr_destroy(r);
-            return 1;
-        }
-        if (ffr_add_buffer(r, in.data(), bytes) != FFR_OK) {
+        if (ffr_add_buffer_stream(r, istream_read, is) != FFR_OK) {
             std::cerr << "ERROR: " << ffr_last_error() << std::endl;
             ffr_destroy(r);
             return 1;
         }
     }
-    std::vector<char>().swap(in);
 
     // render_image (ffr_img.cpp:199-309): bounds, then the pixels
     std::cerr << "processing buffer" << std::endl;

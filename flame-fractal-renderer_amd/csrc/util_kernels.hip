@@ -13,6 +13,7 @@ __global__ void reset_stats_kernel(DevStats *st, unsigned long long *xf_count, i
         st->s_plot = 0;
         st->n_bad = 0;
         st->atomics = 0;
+        st->flush_atomics = 0;
         for (int i = 0; i < 3; ++i) {
             st->pmin[i] = (double)INFINITY;
             st->pmax[i] = -(double)INFINITY;
@@ -34,6 +35,7 @@ __global__ void reset_many_kernel(const ResetEntry *e, int n, unsigned *queues, 
             st->s_plot = 0;
             st->n_bad = 0;
             st->atomics = 0;
+        st->flush_atomics = 0;
             for (int i = 0; i < 3; ++i) {
                 st->pmin[i] = (double)INFINITY;
                 st->pmax[i] = -(double)INFINITY;

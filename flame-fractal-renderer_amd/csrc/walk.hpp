@@ -12,6 +12,7 @@ struct DevStats {
     unsigned long long s_plot;
     unsigned long long n_bad;   // bad values seen; > limit stops new walkers
     unsigned long long atomics; // global atomic adds on the buffer (misses, flushes)
+    unsigned long long flush_atomics; // of those, the LDS cell-cache flushes
     double pmin[3];
     double pmax[3];
 };

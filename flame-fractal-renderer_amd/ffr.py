@@ -59,6 +59,7 @@ class Stats(C.Structure):
         ("pt_max", C.c_double * 3),
         ("kernel_ms", C.c_double),
         ("atomics", C.c_uint64),
+        ("flush_atomics", C.c_uint64),
     ]
 
 
@@ -129,6 +130,11 @@ class ImageInfo(C.Structure):
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 FFR_E_TIMEOUT = 7
 FFR_E_EMPTY = 8
+FFR_E_IO = 9
+
+# ffr_read_fn / ffr_write_fn: size_t (*)(void *ctx, void *buf, size_t bytes)
+READ_FN = C.CFUNCTYPE(C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t)
+WRITE_FN = C.CFUNCTYPE(C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t)
 IMG_MONO, IMG_GRAY, IMG_COLOR = 0, 1, 2
 
 
@@ -139,6 +145,7 @@ EXPORTS = [
     "ffr_device_buffer", "ffr_buffer_cells", "ffr_cell_words", "ffr_word_bytes",
     "ffr_buffer_bytes", "ffr_dims", "ffr_color_dims", "ffr_read_buffer",
     "ffr_add_buffer", "ffr_write_buffer", "ffr_clear_buffer",
+    "ffr_add_buffer_stream", "ffr_read_buffer_stream",
     "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
     "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
     "ffr_histogram_l1", "ffr_choose_precision", "ffr_atomic_rate",
@@ -190,6 +197,8 @@ def lib():
     for name in ("ffr_read_buffer", "ffr_add_buffer", "ffr_write_buffer"):
         getattr(L, name).argtypes = [P, P, C.c_size_t]
     L.ffr_clear_buffer.argtypes = [P]
+    L.ffr_add_buffer_stream.argtypes = [P, READ_FN, P]
+    L.ffr_read_buffer_stream.argtypes = [P, WRITE_FN, P]
     L.ffr_histogram_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]
     L.ffr_xform_distribution.argtypes = [P, C.POINTER(C.c_uint64), C.c_size_t]
@@ -413,6 +422,31 @@ class BufferRenderer:
     def write_buffer(self, words: np.ndarray):
         w = np.ascontiguousarray(words, dtype=self._dtype())
         _check(lib().ffr_write_buffer(self._h, w.ctypes.data_as(C.c_void_p), w.nbytes))
+
+    def add_buffer_stream(self, f):
+        """addBuffer(std::istream&): add a buffer read from a binary file
+        object in chunks, streamed to the device"""
+        def rd(_ctx, dst, n):
+            view = memoryview((C.c_char * n).from_address(dst)).cast("B")
+            if hasattr(f, "readinto"):
+                got = f.readinto(view)
+                return got or 0
+            data = f.read(n)
+            view[: len(data)] = data
+            return len(data)
+        _check(lib().ffr_add_buffer_stream(self._h, READ_FN(rd), None))
+
+    def write_buffer_stream(self, f):
+        """writeBuffer(std::ostream&): write the buffer to a binary file
+        object in chunks, streamed from the device"""
+        def wr(_ctx, src, n):
+            try:
+                # a view of the pinned chunk, no copy on this side
+                f.write(memoryview((C.c_char * n).from_address(src)).cast("B"))
+            except OSError:
+                return 0
+            return n
+        _check(lib().ffr_read_buffer_stream(self._h, WRITE_FN(wr), None))
 
     def add_buffer(self, words: np.ndarray):
         w = np.ascontiguousarray(words, dtype=self._dtype())

@@ -35,7 +35,8 @@ enum {
     FFR_E_SIZE = 5,             /* buffer size mismatch (addBuffer/readBuffer) */
     FFR_E_UNSUPPORTED = 6,      /* flame outside this build's limits           */
     FFR_E_TIMEOUT = 7,          /* a wait ran out of time                      */
-    FFR_E_EMPTY = 8             /* image of an empty histogram                 */
+    FFR_E_EMPTY = 8,            /* image of an empty histogram                 */
+    FFR_E_IO = 9                /* a stream callback came up short             */
 };
 
 /* trajectory precision: num_t of the reference (src/types/types.hpp:29) */
@@ -75,6 +76,7 @@ typedef struct ffr_stats {
     uint64_t atomics;         /* global atomic adds of the last render on the buffer:
                                  one per word of a cell update that missed the LDS
                                  cell cache or flushed it (no reference counterpart)  */
+    uint64_t flush_atomics;   /* of those, the LDS cell-cache flushes                   */
 } ffr_stats;
 
 /* defaults: FP64, WORD64, settle -1, device 0, seed 0, batch 0, limit 256,
@@ -129,6 +131,21 @@ int ffr_add_buffer(ffr_renderer *r, const void *src, size_t bytes);
 /* readBuffer (:457-473): overwrite the buffer */
 int ffr_write_buffer(ffr_renderer *r, const void *src, size_t bytes);
 int ffr_clear_buffer(ffr_renderer *r);
+
+/* Streamed forms of addBuffer(std::istream&) / writeBuffer(std::ostream&)
+ * (buffer_renderer.hpp:420-452, :476-480; ffr-buf -i and -o,
+ * ffr_buf.cpp:167-183, 259-272): the buffer crosses PCIe in chunks of whole
+ * cells through two pinned host buffers, so the caller's reads (or writes)
+ * of one chunk overlap the copy and add (or the copy) of the next.
+ * read(ctx, dst, n) stores up to n bytes and returns how many (0 at the
+ * end); a short buffer fails with FFR_E_IO "error reading file" after the
+ * chunks before it were added, as the reference's stream add leaves a
+ * partial sum.  write(ctx, src, n) returns n on success, else FFR_E_IO
+ * "error writing output". */
+typedef size_t (*ffr_read_fn)(void *ctx, void *dst, size_t bytes);
+typedef size_t (*ffr_write_fn)(void *ctx, const void *src, size_t bytes);
+int ffr_add_buffer_stream(ffr_renderer *r, ffr_read_fn read, void *ctx);
+int ffr_read_buffer_stream(ffr_renderer *r, ffr_write_fn write, void *ctx);
 
 /* histogramSum / histogramMin / histogramMax (:483-507), on the device */
 int ffr_histogram_stats(ffr_renderer *r, uint64_t *sum, uint64_t *min, uint64_t *max);

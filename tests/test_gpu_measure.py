@@ -33,6 +33,15 @@ def test_atomics_with_direct_cache_are_the_flushes(ffr_mod):
     a = r.last_stats.atomics
     assert a % 4 == 0 and a >= occupied * 4
     assert a < r.last_stats.s_plot   # far fewer than one per plot
+    assert r.last_stats.flush_atomics == a   # nothing missed
+    r.close()
+
+
+def test_flush_atomics_without_cache_are_zero(ffr_mod):
+    r = ffr_mod.BufferRenderer(TF.color_flame(r=3, size=64), seed=3, batch_size=512,
+                               cell_cache=0)
+    assert r.render(1 << 20)
+    assert r.last_stats.flush_atomics == 0 and r.last_stats.atomics > 0
     r.close()
 
 

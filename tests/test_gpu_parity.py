@@ -377,3 +377,44 @@ def test_cli_rejects_bad_flames(tmp_path):
     assert rc != 0 and "unknown variation: no_such_variation" in err
     rc, _, err = _run_cli([])
     assert rc == 1 and "--flame" in err
+
+
+@pytest.mark.parametrize("kind", ["color4k", "float_build", "counts3d"])
+def test_streamed_buffer_io(ffr_mod, tmp_path, kind):
+    """ffr_add_buffer_stream / ffr_read_buffer_stream equal the whole-buffer
+    add and read, over several 32 MiB chunks of whole cells"""
+    import io
+    if kind == "color4k":
+        text, kw = read_flame("random_color_3840x2160.json"), {}
+    elif kind == "float_build":
+        text, kw = TF.color_flame(r=3, size=1600), dict(precision="fp32", word=32)
+    else:
+        text, kw = TF.tetrahedron(size=256), {}
+    r = ffr_mod.BufferRenderer(text, seed=11, **kw)
+    assert r.render(1 << 24)
+    whole = r.read_buffer()
+    out = io.BytesIO()
+    r.write_buffer_stream(out)
+    assert out.getvalue() == whole.tobytes()
+    path = tmp_path / "in.buf"
+    path.write_bytes(out.getvalue())
+    with open(path, "rb") as f:
+        r.add_buffer_stream(f)            # the buffer plus itself
+    twice = r.read_buffer()
+    ref = ffr_mod.BufferRenderer(text, seed=11, **kw)
+    ref.write_buffer(whole)
+    ref.add_buffer(whole)
+    assert np.array_equal(twice, ref.read_buffer())
+    # a short stream fails after adding what it held
+    short = io.BytesIO(out.getvalue()[: len(out.getvalue()) // 2 + 7])
+    with pytest.raises(ffr_mod.FfrError, match="error reading file") as e:
+        r.add_buffer_stream(short)
+    assert e.value.code == ffr_mod.FFR_E_IO
+
+    class Full(io.RawIOBase):
+        def write(self, b):
+            raise OSError("no space")
+    with pytest.raises(ffr_mod.FfrError, match="error writing output"):
+        r.write_buffer_stream(Full())
+    r.close()
+    ref.close()
