@@ -437,7 +437,7 @@ def main():
     ap.add_argument("--fp32-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the renderer has no multi-GPU path "
               "(walker ranges would shard across GPUs with one integer sum of the "
               "buffers at the end)", file=sys.stderr)
         return 2

@@ -33,7 +33,7 @@ enum {
     FFR_E_DEVICE = 3,           /* HIP runtime failure                         */
     FFR_E_BADVALUE_LIMIT = 4,   /* render stopped: render() returned false     */
     FFR_E_SIZE = 5,             /* buffer size mismatch (addBuffer/readBuffer) */
-    FFR_E_UNSUPPORTED = 6,      /* flame outside this build's limits           */
+    FFR_E_UNSUPPORTED = 6,      /* flame outside the library's limits          */
     FFR_E_TIMEOUT = 7,          /* a wait ran out of time                      */
     FFR_E_EMPTY = 8,            /* image of an empty histogram                 */
     FFR_E_IO = 9                /* a stream callback came up short             */

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the ffr-buf buffer render -- TEST INFRASTRUCTURE ONLY.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
- * this library, and only as the checker.  See oracle.h for what it restates.
+ * this library, and only to check results against.  See oracle.h for what it restates.
  *
  * Parity status (how each part of this file is pinned; tests named in
  * tests/test_oracle_*.py):

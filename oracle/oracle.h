@@ -3,7 +3,7 @@
  *
  * This is an independent plain-C restatement of the reference renderer's hot
  * path (tkoz0/flame-fractal-renderer), used only by tests/, the smoke check and
- * the CPU-baseline leg of bench.py as the checker.  Nothing in the product
+ * the CPU-baseline leg of bench.py, to check results against.  Nothing in the product
  * links, loads or calls it.
  *
  * What it restates (reference file:line):

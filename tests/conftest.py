@@ -18,7 +18,8 @@ if PKG not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
-    # build only when nothing is built (the GPU box runs what was built here)
+    # build only when nothing is built (a machine without the toolchain runs
+    # what was built before)
     need = [os.path.join(PKG, "libffr_hip.so"), os.path.join(PKG, "ffr-buf"),
             os.path.join(ROOT, "oracle", "liboracle.so")]
     if not all(os.path.exists(p) for p in need):
