@@ -303,3 +303,25 @@ def test_image_edge_values(ffr_mod, oracle_mod, word):
                 # the last bit; double ones move no pixel of these buffers
                 compare(pix, ref, exact=(word == 64 or mode == 0), label=(gamma, mode, bits))
     r.close()
+
+
+def test_ffr_img_pipe_and_float_build(ffr_mod, oracle_mod, tmp_path):
+    """ffr-buf -o - | ffr-img -i -: the buffer streams through both programs;
+    and the float build's layout (--word 32) end to end"""
+    f = tmp_path / "flame.json"
+    f.write_text(TF.color_flame(r=3, size=72).replace("[72, 72]", "[72, 40]"))
+    for word in ("64", "32"):
+        png = tmp_path / f"pipe{word}.png"
+        extra = ["--word", word] + (["--precision", "fp32"] if word == "32" else [])
+        buf = subprocess.run([os.path.join(PKG, "ffr-buf"), "-f", str(f), "-o", "-", "-s",
+                              str(1 << 18), "--seed", "4"] + extra,
+                             capture_output=True, timeout=120)
+        assert buf.returncode == 0, buf.stderr.decode()
+        img = subprocess.run([os.path.join(PKG, "ffr-img"), "-f", str(f), "-i", "-", "-o",
+                              str(png), "-c", "-b", "16", "--word", word],
+                             input=buf.stdout, capture_output=True, timeout=120)
+        assert img.returncode == 0, img.stderr.decode()
+        words = np.frombuffer(buf.stdout, dtype=np.uint64 if word == "64" else np.uint32)
+        st, ref, _, _ = oracle_mod.tonemap(words, 72, 40, 4, 2, 1.0, 16)
+        assert st == 0
+        compare(decode_png(png.read_bytes()), ref, exact=(word == "64"))
