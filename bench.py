@@ -83,14 +83,14 @@ C5_CHUNKS = 4
 C5_BUDGET = 200_000_000_000
 HBM_PEAK_GBS = 8000.0
 # walker kernel HBM bytes per launch on the headline workload (FP32):
-# FETCH_SIZE 894.69 KiB x 2.0 + WRITE_SIZE 18631702.5 KiB x 1.0 per dispatch.
+# FETCH_SIZE 948.75 KiB x 2.0 + WRITE_SIZE 18634055.5 KiB x 1.0 per dispatch.
 # The factors come from kernels of known traffic on the same buffer (the add
-# reads 2 x 265420800 B and reports 259224 KiB of FETCH_SIZE, the histogram
+# reads 2 x 265420800 B and reports 259236 KiB of FETCH_SIZE, the histogram
 # pass reads the buffer once and reports half of it; the clear and the add
 # each write 265420800 B and report exactly 259200 KiB of WRITE_SIZE).
 # Global atomics count 32 B each in WRITE_SIZE.  The raw values are in
 # profiles/walk_kernel_random_color_3840x2160_hbm.md.
-C2_FP32_TRAFFIC_BYTES = 894.69 * 1024 * 2.0 + 18631702.5 * 1024 * 1.0
+C2_FP32_TRAFFIC_BYTES = 948.75 * 1024 * 2.0 + 18634055.5 * 1024 * 1.0
 
 
 def log(*a):
