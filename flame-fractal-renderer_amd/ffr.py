@@ -551,6 +551,9 @@ def histogram_l1(a: "BufferRenderer", b: "BufferRenderer") -> float:
 POLICY_L1_TARGET = 1e-3
 POLICY_PROBE_FRACTION = 1.0 / 64
 POLICY_MARGIN = 0.5
+# the batched policy's second stage: the share of the target its estimate
+# of the full render's L1 must stay below
+POLICY_STAGE2_MARGIN = 0.9
 
 
 def choose_precision(flame_json: str, samples: int, seed: int = 0, batch_size: int = 0,
@@ -583,7 +586,18 @@ def choose_precisions(flame_jsons, samples, seeds, target: float = POLICY_L1_TAR
                       fraction: float = POLICY_PROBE_FRACTION, settle: int = 53):
     """choose_precision for many flames at once: every probe pair (FP32 and
     FP64 on the same streams) renders in one batched launch.  Returns
-    ([("fp32" | "fp64")], [probe L1])."""
+    ([("fp32" | "fp64")], [probe L1]).
+
+    Two stages.  The first probe decides where it is clear: FP64 above the
+    bar of choose_precision, FP32 at or below the target itself (the full
+    render's L1 is at most about the probe's).  Between the two, the L1 may
+    hold a part that shrinks with more samples (drifted walkers are
+    independent draws) and a part that does not (FP32 moves the attractor).
+    For those flames the same renderers render three times the probe again
+    (their streams continue, so the sum is a probe four times larger), and
+    L1(s) = b + k sqrt(s1 / s) is fitted through the two points:
+    k = 2 (L1_1 - L1_2), b = L1_1 - k.  The full render's estimate
+    b + k sqrt(s1 / n) must stay below POLICY_STAGE2_MARGIN * target."""
     L = lib()
     rs, bars = [], []
     try:
@@ -596,13 +610,30 @@ def choose_precisions(flame_jsons, samples, seeds, target: float = POLICY_L1_TAR
                                          settle=settle))
         with Batch(rs) as b:
             ok = b.render([p for p, _ in bars for _ in range(2)])
-        out, dist = [], []
-        for i, (_, bar) in enumerate(bars):
-            r32, r64 = rs[2 * i], rs[2 * i + 1]
-            d = histogram_l1(r32, r64)
-            escapes = (not ok[2 * i]) and ok[2 * i + 1]
-            out.append("fp32" if (d <= bar and not escapes) else "fp64")
-            dist.append(d)
+            out, dist, second = [], [], []
+            for i, (_, bar) in enumerate(bars):
+                d = histogram_l1(rs[2 * i], rs[2 * i + 1])
+                escapes = (not ok[2 * i]) and ok[2 * i + 1]
+                if escapes or d > bar:
+                    out.append("fp64")
+                elif d <= target:
+                    out.append("fp32")
+                else:
+                    out.append(None)
+                    second.append(i)
+                dist.append(d)
+            if second:
+                more = [0] * len(rs)
+                for i in second:
+                    more[2 * i] = more[2 * i + 1] = 3 * bars[i][0]
+                ok2 = b.render(more)
+                for i in second:
+                    d2 = histogram_l1(rs[2 * i], rs[2 * i + 1])
+                    k = 2.0 * (dist[i] - d2)
+                    est = (dist[i] - k) + k * (bars[i][0] / max(samples[i], 1)) ** 0.5
+                    escapes = (not ok2[2 * i]) and ok2[2 * i + 1]
+                    out[i] = ("fp32" if (not escapes and est <= POLICY_STAGE2_MARGIN * target)
+                              else "fp64")
         return out, dist
     finally:
         for r in rs:

@@ -192,3 +192,22 @@ def test_precision_policy(ffr_mod):
     assert prec == "fp32" and d < 1e-3, d
     prec, d = ffr_mod.choose_precision(chaotic(), 1 << 26, seed=3)
     assert prec == "fp64" and d > 4e-3, d
+
+
+def test_precision_policy_two_stages(ffr_mod):
+    """the batched policy: clear cases decided by the first probe, the band
+    between target and bar by the second (forced here by a target of half
+    the chaotic flame's first-probe L1)"""
+    sier = read_flame("sierpinski_1024.json").replace("[1024, 1024]", "[64, 64]")
+    texts = [sier, chaotic()]
+    n = [1 << 26, 1 << 26]
+    prec, d = ffr_mod.choose_precisions(texts, n, [3, 3])
+    assert prec[0] == "fp32" and d[0] < 1e-3
+    assert prec[1] == "fp64" and d[1] > 4e-3
+    # the same first probe with a target that puts the chaotic flame in
+    # the band (target, 4 target]: the second stage decides it
+    prec2, d2 = ffr_mod.choose_precisions(texts, n, [3, 3], target=d[1] / 2)
+    # the first probe is the same render (its L1 is summed with atomics,
+    # so only the last bits may differ)
+    assert d2 == pytest.approx(d, rel=1e-12)
+    assert prec2[0] == "fp32" and prec2[1] in ("fp32", "fp64")
