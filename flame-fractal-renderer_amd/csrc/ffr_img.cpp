@@ -11,6 +11,7 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -214,7 +215,7 @@ void put_chunk(std::string &png, const char *type, const unsigned char *data, si
     if (n)
         png.append((const char *)data, n);
     uLong crc = crc32(0L, Z_NULL, 0);
-    crc = crc32(crc, (const Bytef *)png.data() + start, (uInt)(4 + n));
+    crc = crc32_z(crc, (const Bytef *)png.data() + start, (z_size_t)(4 + n));
     put_u32(png, (uint32_t)crc);
 }
 
@@ -299,7 +300,11 @@ bool encode_png(const void *pixels, uint32_t w, uint32_t h, uint32_t channels,
     ihdr[11] = 0;                                    // adaptive filtering
     ihdr[12] = 0;                                    // no interlace
     put_chunk(png, "IHDR", ihdr, sizeof ihdr);
-    put_chunk(png, "IDAT", z.data(), zlen);
+    // a chunk holds at most 2^31 - 1 bytes: large images take several
+    // IDAT chunks, whose data concatenate into the one zlib stream
+    const size_t max_chunk = (size_t)1 << 30;
+    for (size_t off = 0; off < (size_t)zlen; off += max_chunk)
+        put_chunk(png, "IDAT", z.data() + off, std::min(max_chunk, (size_t)zlen - off));
     put_chunk(png, "IEND", nullptr, 0);
     return true;
 }
