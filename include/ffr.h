@@ -302,7 +302,7 @@ enum { FFR_JOB_QUEUED = 0, FFR_JOB_RUNNING = 1, FFR_JOB_DONE = 2, FFR_JOB_FAILED
 
 typedef struct ffr_server_options {
     uint64_t hbm_budget;   /* bytes of job buffers held at once (0: 200e9)          */
-    double pass_ms;        /* target device time of one pass, ms (0: 40)            */
+    double pass_ms;        /* target device time of one pass, ms (0: 100)           */
     int32_t device;        /* HIP device ordinal                                    */
     int32_t reserved;
 } ffr_server_options;
