@@ -211,3 +211,26 @@ def test_precision_policy_two_stages(ffr_mod):
     # so only the last bits may differ)
     assert d2 == pytest.approx(d, rel=1e-12)
     assert prec2[0] == "fp32" and prec2[1] in ("fp32", "fp64")
+
+
+def test_hot_cell_hints_leave_results_alone(ffr_mod):
+    """40 flames of 256^2 hashed-cache cells, 16 tasks each: every flame has
+    at most ~6 workgroups on it at a time, so its tasks seed their caches
+    from the hints its earlier tasks left (written while other tasks of the
+    same flame read them).  Counts equal the same renders done alone, which
+    use no hints; colours agree to rounding."""
+    flames = [TF.color_flame(3, size=256, seed=200 + i) for i in range(40)]
+    n = 1 << 22
+    batched = [ffr_mod.BufferRenderer(t, precision="fp32", seed=300 + i, batch_size=256)
+               for i, t in enumerate(flames)]
+    alone = [ffr_mod.BufferRenderer(t, precision="fp32", seed=300 + i, batch_size=256)
+             for i, t in enumerate(flames)]
+    with ffr_mod.Batch(batched) as b:
+        for rep in range(2):          # the second pass starts from the first's hints
+            assert all(b.render([n] * len(batched)))
+            for r in alone:
+                assert r.render(n)
+            for x, y in zip(batched, alone):
+                assert_same(x, y)
+    for r in batched + alone:
+        r.close()
