@@ -81,6 +81,13 @@ template <typename R> __device__ __forceinline__ R dev_norm2sq(const R *x, int n
 // evaluate for it (variations.hpp:169-208, 509-555), with the squared norm
 // and its root shared.  `need` has bit k set when some lane of the wave
 // uses code k; the others are left untouched.
+// 1.0 / x as the reference writes it for spherical and horseshoe: with
+// num_t = float that is a double division whose quotient is then rounded to
+// float.  Double rounding of a quotient is innocuous when the wide format
+// has at least 2p + 2 bits (Figueroa 1995; 53 >= 2 * 24 + 2), so the
+// correctly rounded float division gives the same bits, at float cost.
+template <typename R> __device__ __forceinline__ R recip_ref(R x) { return (R)1 / x; }
+
 template <typename R>
 __device__ __forceinline__ void basic6(unsigned need, const R *t, R (&o)[6][2])
 {
@@ -94,7 +101,7 @@ __device__ __forceinline__ void basic6(unsigned need, const R *t, R (&o)[6][2])
         o[1][1] = m_sin(y);
     }
     if (need & 4u) {
-        R r = 1.0 / (n2sq + EPS);
+        R r = recip_ref(n2sq + EPS);
         o[2][0] = x * r;
         o[2][1] = y * r;
     }
@@ -108,7 +115,7 @@ __device__ __forceinline__ void basic6(unsigned need, const R *t, R (&o)[6][2])
     if (need & 48u)
         n2 = (R)m_sqrt(n2sq);
     if (need & 16u) {
-        R r = 1.0 / (n2 + EPS);
+        R r = recip_ref(n2 + EPS);
         R a = (x - y) * (x + y);
         R b = 2.0 * x * y;
         o[4][0] = a * r;
@@ -137,7 +144,7 @@ __device__ __forceinline__ void calc2d(int type, const R *p, R x, R y, R &ox, R 
         oy = x * cr + y * sr;
     } break;
     case 21: if constexpr (vs_has(VS, 21)) { // horseshoe
-        R r = 1.0 / (N2 + EPS);
+        R r = recip_ref(N2 + EPS);
         R a = (x - y) * (x + y);
         R b = 2.0 * x * y;
         ox = a * r;
@@ -807,7 +814,7 @@ __device__ __forceinline__ void var_eval(int type, const R *E, const R *t, R *ou
             out[i] = m_sin(t[i]);
     } break;
     case 2: if constexpr (vs_has(VS, 2)) { // spherical
-        R r = 1.0 / (dev_norm2sq(t, D) + EPS);
+        R r = recip_ref(dev_norm2sq(t, D) + EPS);
 #pragma unroll
         for (int i = 0; i < D; ++i)
             out[i] = t[i] * r;

@@ -418,3 +418,27 @@ def test_streamed_buffer_io(ffr_mod, tmp_path, kind):
         r.write_buffer_stream(Full())
     r.close()
     ref.close()
+
+
+@pytest.mark.parametrize("name,dims", [("spherical", 2), ("horseshoe", 2), ("spherical", 3),
+                                       ("linear", 2)])
+@pytest.mark.parametrize("mode", ["fp32u64", "fp32", "fp64"])
+def test_division_variations_bit_exact(ffr_mod, oracle_mod, name, dims, mode):
+    """spherical and horseshoe hold only +, x, / and sqrt, all correctly
+    rounded on both sides.  The device divides in num_t where the reference
+    (and the oracle) divides 1.0 by a num_t in double and rounds back: the
+    same bits, since double rounding of a quotient is innocuous when the
+    wide format has 2p + 2 bits.  So the images agree bit for bit."""
+    text = TF.single_variation_flame(name, dims)
+    f = oracle_mod.OracleFlame(text, MODES[mode], settle=0)
+    word = 32 if mode == "fp32" else 64
+    for k in range(48):
+        r = ffr_mod.BufferRenderer(text, precision="fp64" if mode == "fp64" else "fp32",
+                                   word=word, seed=SEED + 1000 + k, batch_size=1, settle=0,
+                                   bad_value_limit=10 ** 6)
+        assert r.render(1)
+        ref = f.render(1, 1, seed=SEED + 1000 + k, bv_limit=10 ** 6, threads=1)
+        g = [v for pair in r.point_extremes for v in pair]
+        o = [v for i in range(dims) for v in (ref.pt_min[i], ref.pt_max[i])]
+        r.close()
+        assert g == o, (name, dims, mode, k, g, o)
