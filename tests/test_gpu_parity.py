@@ -442,3 +442,46 @@ def test_division_variations_bit_exact(ffr_mod, oracle_mod, name, dims, mode):
         o = [v for i in range(dims) for v in (ref.pt_min[i], ref.pt_max[i])]
         r.close()
         assert g == o, (name, dims, mode, k, g, o)
+
+
+def _mixed_flame(seed):
+    """3 xforms of two variations each, drawn from all 98 (2-D), with
+    colours: the general variation-slot path, random draws included"""
+    import oracle.model as M
+    rng = np.random.default_rng(seed)
+    names = [n for n, _ in M.VARIATIONS]
+    xfs = []
+    for _ in range(3):
+        vs = [TF.variation(names[int(i)], 0.5, 2)
+              for i in rng.choice(len(names), 2, replace=False)]
+        xfs.append({"weight": float(rng.uniform(0.5, 1.5)), "variations": vs,
+                    "pre_affine": {"A": (rng.uniform(-1, 1, (2, 2)) * 0.4).tolist(),
+                                   "b": rng.uniform(-0.5, 0.5, 2).tolist()},
+                    "color": rng.uniform(0, 1, 3).tolist()})
+    return json.dumps({"dimensions": 2, "size": [64, 64], "bounds": [[-3, 3], [-3, 3]],
+                       "color_dimensions": 3, "xforms": xfs})
+
+
+def test_mixed_variation_flames_whole_render(ffr_mod, oracle_mod):
+    """24 random flames over the whole catalogue, 2^20 samples each, FP64:
+    the GPU render against the oracle on the same streams.  Where every map
+    holds only correctly rounded operations, or its libm calls agree, the
+    histograms are equal; chaotic maps amplify a last-bit libm difference,
+    so the bar is 1e-3 or a tenth of the L1 between two oracle renders on
+    different streams, whichever is larger.  Most flames must be equal."""
+    n, batch = 1 << 20, 1024
+    equal = 0
+    for seed in range(24):
+        text = _mixed_flame(seed)
+        r = ffr_mod.BufferRenderer(text, seed=7, batch_size=batch, bad_value_limit=10 ** 9)
+        r.render(n)
+        g = r.counts()
+        r.close()
+        f = oracle_mod.OracleFlame(text, oracle_mod.F64_U64)
+        ref = f.render(n, batch, seed=7, threads=8, bv_limit=10 ** 9).counts
+        other = f.render(n, batch, seed=8, threads=8, bv_limit=10 ** 9).counts
+        d, floor = l1(g, ref), l1(ref, other)
+        print(f"flame {seed}: L1 {d:.2e}, floor {floor:.2e}", flush=True)
+        assert d <= max(1e-3, 0.1 * floor), (seed, d, floor)
+        equal += bool(np.array_equal(g, ref))
+    assert equal >= 18, equal
