@@ -89,14 +89,16 @@ struct Stream {
 
 // randNum (flame_rng.hpp:67-87): top 53 bits over 2^53 in double, top 24
 // bits over 2^24 in float; both exact
-__device__ __forceinline__ double rand_num(Stream &s, double)
+__device__ __forceinline__ double rand_from_word(uint64_t w, double)
 {
-    return (double)(s.next() >> 11) * 0x1p-53;
+    return (double)(w >> 11) * 0x1p-53;
 }
-__device__ __forceinline__ float rand_num(Stream &s, float)
+__device__ __forceinline__ float rand_from_word(uint64_t w, float)
 {
-    return (float)(s.next() >> 40) * 0x1p-24f;
+    return (float)(w >> 40) * 0x1p-24f;
 }
+__device__ __forceinline__ double rand_num(Stream &s, double d) { return rand_from_word(s.next(), d); }
+__device__ __forceinline__ float rand_num(Stream &s, float f) { return rand_from_word(s.next(), f); }
 
 // ---------------------------------------------------------------------------
 // math overloads: the argument type picks the precision (C++ <cmath> rules)

@@ -62,6 +62,9 @@ struct ffr_renderer {
     DeviceFlame<float> f32;
     DevTables tab;
     int vs = 2;
+    // the wave-per-walker kernel applies: linear variations only (at most
+    // kChainSlots per xform), no final xform, no colour
+    bool chain_ok = false;
     size_t lds_bytes = 0;
     // buffer
     void *dbuf = nullptr;
@@ -110,6 +113,7 @@ struct ffr_renderer {
     size_t max_lds = 65536;
     size_t lds_per_cu = 65536;
     int last_cache = 0;
+    bool last_chain = false;
 };
 
 static int pick_variation_set(const uint32_t ts[4])
@@ -152,6 +156,8 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
                                            " bytes exceeds the " + std::to_string(max_lds) +
                                            "-byte LDS of a workgroup");
     r->vs = pick_variation_set(d.type_set);
+    r->chain_ok = r->vs == 0 /* linear only */ && !d.has_final && d.r == 0 &&
+                  d.slot_type.size() <= (size_t)kChainSlots;
     r->nxf = d.nxf;
     r->order = d.order;
     return FFR_OK;
@@ -407,6 +413,7 @@ static LaunchShape renderer_shape(const ffr_renderer *r)
 
 // launch geometry of one render pass (see plan_launch)
 struct LaunchPlan {
+    bool chain = false; // one walker per wave (chain_kernel)
     bool cache = false;
     int block = 256;
     int grid = 1;
@@ -432,6 +439,18 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
 {
     LaunchPlan p;
     const size_t base = std::max(table_lds(r), base_lds);
+    // few walkers of a linear flame: one walker per wave, its chain split
+    // into parallel draws and plots around a short serial part (unless the
+    // cache or the workgroup size is forced)
+    if (sh.cache < 0 && sh.block <= 0 && base_lds == 0 && r->chain_ok &&
+        walkers <= (uint64_t)r->cu_count * (uint64_t)kChainWalkersPerCu) {
+        p.chain = true;
+        p.block = 64 * kChainWaves;
+        p.task_walkers = (uint64_t)kChainWaves;
+        p.grid = (int)((walkers + kChainWaves - 1) / kChainWaves);
+        p.lds = base;
+        return p;
+    }
     const int colour_bytes = r->word_bytes;
     const size_t slot_bytes = cache_slot_bytes(r->spec.color_dims, colour_bytes);
     bool want = sh.cache != 0 && r->cells < 0xFFFFFFFFull && r->spec.color_dims <= 4;
@@ -550,6 +569,7 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     WalkLaunch L;
     L.nc = r->spec.color_dims;
     L.vs = r->vs;
+    L.chain = p.chain;
     L.cache = p.cache;
     L.grid = p.grid;
     L.block = p.block;
@@ -576,6 +596,7 @@ static int launch(ffr_renderer *r, uint64_t samples)
     r->last_walkers = walkers;
     r->last_batch = batch;
     r->last_cache = p.cache ? p.slots : 0;
+    r->last_chain = p.chain;
     int rc = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream);
     if (rc)
         return fail(FFR_E_DEVICE, std::string("stats reset: ") + hipGetErrorString((hipError_t)rc));
@@ -645,6 +666,8 @@ static int finish(ffr_renderer *r, ffr_stats *out)
         out->kernel_ms = r->last_ms;
         out->atomics = r->last_atomics;
         out->flush_atomics = r->last_flush_atomics;
+        out->wave_per_walker = r->last_chain ? 1u : 0u;
+        out->cache_slots = (uint32_t)r->last_cache;
     }
     if (r->last_bad > r->opt.bad_value_limit)
         return fail(FFR_E_BADVALUE_LIMIT, "render failure (flame may not be sufficiently contractive)");

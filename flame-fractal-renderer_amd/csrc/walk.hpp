@@ -86,6 +86,13 @@ struct BatchTask {
 // hot-cell hint slots kept per renderer (caches with more slots take none)
 constexpr int kHotSlots = 1 << 15;
 
+// the wave-per-walker kernel (chain_kernel.inc): variation slots per xform
+// it holds, walkers (waves) per workgroup, and the most walkers per CU it
+// takes (beyond, the lane-per-walker kernel has enough waves in flight)
+constexpr int kChainSlots = 4;
+constexpr int kChainWaves = 4;
+constexpr int kChainWalkersPerCu = 16;
+
 // a walker-kernel launch: one flame (args, a device WalkArgs) or a batch
 // (descs, device WalkArgs[] indexed by the tasks' flame numbers)
 struct WalkLaunch {
@@ -100,6 +107,7 @@ struct WalkLaunch {
     const unsigned *ntasks = nullptr; // device word: the number of tasks
     unsigned *queue = nullptr;        // device word: next task (zero at launch)
     bool prepare_only = false;        // set the kernel's LDS limit, launch nothing
+    bool chain = false;               // one flame, one walker per wave (chain_kernel)
 };
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;

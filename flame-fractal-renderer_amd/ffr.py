@@ -60,6 +60,8 @@ class Stats(C.Structure):
         ("kernel_ms", C.c_double),
         ("atomics", C.c_uint64),
         ("flush_atomics", C.c_uint64),
+        ("wave_per_walker", C.c_uint32),
+        ("cache_slots", C.c_uint32),
     ]
 
 
@@ -321,6 +323,17 @@ class DeviceMemory:
                                      self.nbytes, 2) != 0:  # device to host
             raise FfrError(3, "hipMemcpy failed")
         return out.view(dtype)
+
+    @classmethod
+    def cu_count(cls, device: int = 0) -> int:
+        """compute units of a device (hipDeviceAttributeMultiprocessorCount)"""
+        h = cls._runtime()
+        v = C.c_int()
+        h.hipDeviceGetAttribute.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
+        # hipDeviceAttributeMultiprocessorCount
+        if h.hipDeviceGetAttribute(C.byref(v), 63, device) != 0:
+            raise FfrError(3, "hipDeviceGetAttribute failed")
+        return v.value
 
     def close(self):
         if getattr(self, "ptr", None):

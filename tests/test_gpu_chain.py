@@ -1,0 +1,219 @@
+"""The wave-per-walker kernel (few walkers of a linear flame, no final xform,
+no colour) against the lane-per-walker kernel and the oracle.
+
+A render with the cell cache left to the renderer (-1) and fewer walkers
+than 16 per CU takes the wave-per-walker kernel; `cell_cache=0` forces the
+lane-per-walker kernel on the same walkers and streams.  Buffers, samples
+iterated and plotted, xform counts, point extremes and bad values must be
+equal bit for bit, in every precision mode and at walker shapes that leave
+partial blocks of 64 iterations (short walkers, a short last walker).
+"""
+
+import json
+import re
+
+import numpy as np
+import pytest
+
+import flames as TF
+from conftest import read_flame
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x5EED0007
+MODES = {"fp64": 0, "fp32": 1, "fp32u64": 2}
+
+
+def sierpinski(size=128):
+    return read_flame("sierpinski_1024.json").replace("[1024, 1024]", f"[{size}, {size}]")
+
+
+def load(text):
+    """a flame document (the shipped files carry // comment lines)"""
+    return json.loads(re.sub(r"^\s*//.*$", "", text, flags=re.M))
+
+
+def three_slots(seed=11, size=96):
+    """four maps, each the sum of three linear variations of random weight"""
+    rng = np.random.default_rng(seed)
+    xfs = []
+    for k in range(4):
+        A = (rng.uniform(-0.6, 0.6, (2, 2))).tolist()
+        w = rng.dirichlet([1, 1, 1]).tolist()
+        xfs.append({"weight": float(rng.uniform(0.5, 2)),
+                    "variations": [{"name": "linear", "weight": x} for x in w],
+                    "pre_affine": {"A": A, "b": rng.uniform(-1, 1, 2).tolist()},
+                    "post_affine": {"A": [[1, 0.2], [-0.1, 0.9]],
+                                    "b": rng.uniform(-0.2, 0.2, 2).tolist()}})
+    return json.dumps({"dimensions": 2, "size": [size, size], "bounds": [[-3, 3], [-3, 3]],
+                       "xforms": xfs})
+
+
+def bad_no_final(size=64):
+    """two halving maps and a rare one that throws the point beyond 1e20
+    (a bad value: the walker starts again)"""
+    return json.dumps({
+        "dimensions": 2, "size": [size, size], "bounds": [[-2, 2], [-2, 2]],
+        "xforms": [
+            {"weight": 1.0, "variations": [{"name": "linear", "weight": 1.0}],
+             "pre_affine": {"A": [[0.5, 0], [0, 0.5]], "b": [0.5, 0]}},
+            {"weight": 1.0, "variations": [{"name": "linear", "weight": 1.0}],
+             "pre_affine": {"A": [[0.5, 0], [0, 0.5]], "b": [-0.5, 0.3]}},
+            {"weight": 0.004, "variations": [{"name": "linear", "weight": 1.0}],
+             "pre_affine": {"A": [[1e25, 0], [0, 1e25]], "b": [0, 0]}},
+        ]})
+
+
+FLAMES = {
+    "sierpinski": (sierpinski, 2),
+    "fern": (TF.fern, 2),
+    "tetrahedron": (TF.tetrahedron, 3),
+    "tetrahedron_post": (lambda: TF.tetrahedron(post=True), 3),
+    "one_d": (TF.one_d, 1),
+    "tied_weights": (TF.tied_weights, 2),
+    "three_slots": (three_slots, 2),
+}
+
+
+def render(ffr, text, samples, batch, mode, cache, settle=-1, seed=SEED, **kw):
+    precision = "fp64" if mode == "fp64" else "fp32"
+    word = 32 if mode == "fp32" else 64
+    r = ffr.BufferRenderer(text, precision=precision, word=word, seed=seed,
+                           batch_size=batch, settle=settle, cell_cache=cache, **kw)
+    ok = r.render(samples)
+    return r, ok
+
+
+def records(ids, pts):
+    """bad-value records as a sorted list; NaN coordinates (FP32 points that
+    overflowed) compare equal to each other"""
+    def coord(v):
+        return (1, 0.0) if np.isnan(v) else (0, float(v))
+    return sorted((int(i), tuple(coord(v) for v in row)) for i, row in zip(ids, pts))
+
+
+def assert_same(a, b, dims):
+    assert np.array_equal(a.read_buffer(), b.read_buffer())
+    assert a.samples_iterated == b.samples_iterated
+    assert a.samples_plotted == b.samples_plotted
+    assert np.array_equal(a.xform_distribution(), b.xform_distribution())
+    ea, eb = a.point_extremes, b.point_extremes
+    for i in range(dims):
+        assert ea[i] == eb[i], (i, ea, eb)
+    assert a.last_stats.n_bad == b.last_stats.n_bad
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "fp32u64"])
+@pytest.mark.parametrize("name", sorted(FLAMES))
+def test_chain_equals_lane_kernel(ffr_mod, name, mode):
+    maker, dims = FLAMES[name]
+    text = maker()
+    # 101 walkers, the last of 3 samples; then short walkers (37 samples,
+    # under one block of 64), with settle iterations
+    for samples, batch, settle in ((100_003, 1000, -1), (5_032, 37, 20)):
+        print(f"{name} {mode} samples {samples} batch {batch}", flush=True)
+        a, oka = render(ffr_mod, text, samples, batch, mode, -1, settle)
+        b, okb = render(ffr_mod, text, samples, batch, mode, 0, settle)
+        assert oka and okb
+        assert a.last_stats.wave_per_walker == 1
+        assert b.last_stats.wave_per_walker == 0
+        assert_same(a, b, dims)
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_chain_matches_oracle(ffr_mod, oracle_mod, mode):
+    text = three_slots()
+    samples, batch = 200_000, 2000
+    r, ok = render(ffr_mod, text, samples, batch, mode, -1, settle=20)
+    assert ok and r.last_stats.wave_per_walker == 1
+    f = oracle_mod.OracleFlame(text, MODES[mode], settle=20)
+    ref = f.render(samples, batch, seed=SEED, threads=8)
+    assert ref.success
+    assert np.array_equal(r.read_buffer(), ref.buffer)
+    assert r.samples_iterated == ref.s_iter and r.samples_plotted == ref.s_plot
+    assert np.array_equal(r.xform_distribution(), ref.xf_dist)
+    ext = r.point_extremes
+    for i in range(2):
+        assert ext[i] == (ref.pt_min[i], ref.pt_max[i])
+    r.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32u64"])
+def test_chain_bad_values_restart_exact(ffr_mod, oracle_mod, mode):
+    """bad values end a block: the walker starts again from the next word;
+    every bad point, its xform, the counts and extremes equal the oracle's"""
+    text = bad_no_final()
+    samples, batch = 300_000, 1500
+    r, ok = render(ffr_mod, text, samples, batch, mode, -1, bad_value_limit=10 ** 9)
+    assert ok and r.last_stats.wave_per_walker == 1
+    f = oracle_mod.OracleFlame(text, MODES[mode], settle=-1)
+    ref = f.render(samples, batch, seed=SEED, threads=1, bv_limit=10 ** 9, bv_cap=4096)
+    assert ref.success and ref.n_bad > 200
+    assert np.array_equal(r.read_buffer(), ref.buffer)
+    assert r.samples_iterated == ref.s_iter and r.samples_plotted == ref.s_plot
+    assert np.array_equal(r.xform_distribution(), ref.xf_dist)
+    ids, pts = r.bad_values()
+    assert len(ids) == ref.n_bad
+    assert records(ids, pts) == records(ref.bad_xforms, ref.bad_points)
+    lane, _ = render(ffr_mod, text, samples, batch, mode, 0, bad_value_limit=10 ** 9)
+    assert_same(r, lane, 2)
+    r.close()
+    lane.close()
+
+
+def test_chain_bad_value_limit_stops(ffr_mod):
+    text = TF.expanding()
+    r, ok = render(ffr_mod, text, 100_000, 1000, "fp64", -1, bad_value_limit=0)
+    assert not ok and r.last_stats.n_bad >= 1
+    assert r.last_stats.wave_per_walker == 1
+    r.close()
+
+
+def test_chain_at_walker_limit_and_cumulative(ffr_mod):
+    """the most walkers the kernel takes (16 per CU), then a second render
+    onto the same buffer: cumulative statistics equal the lane kernel's"""
+    text = sierpinski(256)
+    a = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=300)
+    b = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=300, cell_cache=0)
+    walkers = 16 * ffr_mod.DeviceMemory.cu_count()
+    for rep in range(2):
+        print(f"walker limit {walkers}, render {rep}", flush=True)
+        assert a.render(walkers * 300) and b.render(walkers * 300)
+        assert a.last_stats.walkers == walkers
+        assert a.last_stats.wave_per_walker == 1
+        assert_same(a, b, 2)
+    assert a.render((walkers + 1) * 300)
+    assert a.last_stats.wave_per_walker == 0
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("case", ["final", "colour", "cache", "block", "five_slots",
+                                  "sinusoidal"])
+def test_chain_not_taken(ffr_mod, case):
+    text, kw = sierpinski(64), {}
+    if case == "final":
+        text = TF.rotations()
+    elif case == "colour":
+        d = load(text)
+        d["color_dimensions"] = 1
+        for x in d["xforms"]:
+            x["color"] = [0.5]
+        text = json.dumps(d)
+    elif case == "cache":
+        kw["cell_cache"] = 1
+    elif case == "block":
+        kw["block"] = 64
+    elif case == "five_slots":
+        d = load(text)
+        for x in d["xforms"]:
+            x["variations"] = [{"name": "linear", "weight": 0.2}] * 5
+        text = json.dumps(d)
+    else:
+        text = TF.one_d(nonlinear=True)
+    r = ffr_mod.BufferRenderer(text, seed=1, batch_size=1000, **kw)
+    assert r.render(50_000)
+    assert r.last_stats.wave_per_walker == 0
+    r.close()

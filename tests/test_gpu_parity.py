@@ -126,6 +126,23 @@ def test_linear_flames_bit_exact(ffr_mod, oracle_mod, name, maker, dims, mode, c
     r.close()
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_xform_counts_with_idle_lanes(ffr_mod, oracle_mod, mode):
+    """More xforms than byte-packed counters (24, counted per wave), walkers
+    of 37 samples and a last task of 8 walkers: lanes whose walkers are done
+    sit idle while the others count, and no selection may be lost."""
+    text = TF.tied_weights()
+    samples, batch = 5_032, 37
+    r, ok = gpu_render(ffr_mod, text, samples, batch, mode, settle=20, cell_cache=0)
+    ref = oracle_render(oracle_mod, text, samples, batch, mode, settle=20, threads=1)
+    assert ok and ref.success
+    assert r.last_stats.walkers == 136
+    assert int(r.xform_distribution().sum()) == samples
+    assert np.array_equal(r.read_buffer(), ref.buffer)
+    assert_same_stats(r, ref, 2)
+    r.close()
+
+
 @pytest.mark.parametrize("r_dims,final_color", [(3, True), (3, False), (1, True), (5, True)])
 @pytest.mark.parametrize("mode", ["fp64", "fp32u64", "fp32"])
 @pytest.mark.parametrize("cache", CACHE)
