@@ -570,6 +570,7 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     L.nc = r->spec.color_dims;
     L.vs = r->vs;
     L.chain = p.chain;
+    L.chain_slots = (int)d.slot_type.size();
     L.cache = p.cache;
     L.grid = p.grid;
     L.block = p.block;

@@ -108,6 +108,7 @@ struct WalkLaunch {
     unsigned *queue = nullptr;        // device word: next task (zero at launch)
     bool prepare_only = false;        // set the kernel's LDS limit, launch nothing
     bool chain = false;               // one flame, one walker per wave (chain_kernel)
+    int chain_slots = 0;              // its variation slots (1..kChainSlots)
 };
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;
