@@ -65,6 +65,7 @@ struct ffr_renderer {
     // the wave-per-walker kernel applies: linear variations only (at most
     // kChainSlots per xform), no final xform, no colour
     bool chain_ok = false;
+    bool chain_post_id = false; // ... and every post-affine is the identity
     size_t lds_bytes = 0;
     // buffer
     void *dbuf = nullptr;
@@ -158,6 +159,22 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
     r->vs = pick_variation_set(d.type_set);
     r->chain_ok = r->vs == 0 /* linear only */ && !d.has_final && d.r == 0 &&
                   d.slot_type.size() <= (size_t)kChainSlots;
+    // every xform's post-affine is exactly the identity (entries 1.0 and
+    // +0.0, bit for bit): the wave-per-walker kernel then applies it with
+    // constants, the same operations on the same values
+    r->chain_post_id = true;
+    for (int x = 0; x < d.nxf && r->chain_post_id; ++x) {
+        const R *X = d.table.data() + d.xf0 + (size_t)x * (size_t)d.lay.stride;
+        for (int i = 0; i < d.dims * d.dims; ++i) {
+            const R v = X[d.lay.postA + i];
+            const R want = (i / d.dims == i % d.dims) ? (R)1 : (R)0;
+            if (!(v == want && !std::signbit(v)))
+                r->chain_post_id = false;
+        }
+        for (int i = 0; i < d.dims; ++i)
+            if (!(X[d.lay.postb + i] == (R)0 && !std::signbit(X[d.lay.postb + i])))
+                r->chain_post_id = false;
+    }
     r->nxf = d.nxf;
     r->order = d.order;
     return FFR_OK;
@@ -571,6 +588,7 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     L.vs = r->vs;
     L.chain = p.chain;
     L.chain_slots = (int)d.slot_type.size();
+    L.chain_post_id = r->chain_post_id;
     L.cache = p.cache;
     L.grid = p.grid;
     L.block = p.block;

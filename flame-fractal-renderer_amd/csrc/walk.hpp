@@ -109,6 +109,7 @@ struct WalkLaunch {
     bool prepare_only = false;        // set the kernel's LDS limit, launch nothing
     bool chain = false;               // one flame, one walker per wave (chain_kernel)
     int chain_slots = 0;              // its variation slots (1..kChainSlots)
+    bool chain_post_id = false;       // every xform's post-affine is the identity
 };
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;

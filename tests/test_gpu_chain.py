@@ -1,5 +1,7 @@
 """The wave-per-walker kernel (few walkers of a linear flame, no final xform,
-no colour) against the lane-per-walker kernel and the oracle.
+no colour) against the lane-per-walker kernel and the oracle.  Flames with
+identity post-affines take its constant-post build, the others (three_slots,
+negzero_post) read the post-affine from the table.
 
 A render with the cell cache left to the renderer (-1) and fewer walkers
 than 16 per CU takes the wave-per-walker kernel; `cell_cache=0` forces the
@@ -64,8 +66,18 @@ def bad_no_final(size=64):
         ]})
 
 
+def negzero_post(size=96):
+    """Sierpinski with post-affines whose offsets are -0.0: not the identity
+    bit for bit (-0 + +0 is +0), so the kernel reads them from the table"""
+    d = load(sierpinski(size))
+    for x in d["xforms"]:
+        x["post_affine"] = {"A": [[1, 0], [0, 1]], "b": [-0.0, -0.0]}
+    return json.dumps(d)
+
+
 FLAMES = {
     "sierpinski": (sierpinski, 2),
+    "negzero_post": (negzero_post, 2),
     "fern": (TF.fern, 2),
     "tetrahedron": (TF.tetrahedron, 3),
     "tetrahedron_post": (lambda: TF.tetrahedron(post=True), 3),
@@ -122,9 +134,10 @@ def test_chain_equals_lane_kernel(ffr_mod, name, mode):
         b.close()
 
 
+@pytest.mark.parametrize("flame", ["three_slots", "negzero_post", "sierpinski"])
 @pytest.mark.parametrize("mode", ["fp64", "fp32"])
-def test_chain_matches_oracle(ffr_mod, oracle_mod, mode):
-    text = three_slots()
+def test_chain_matches_oracle(ffr_mod, oracle_mod, mode, flame):
+    text = FLAMES[flame][0]()
     samples, batch = 200_000, 2000
     r, ok = render(ffr_mod, text, samples, batch, mode, -1, settle=20)
     assert ok and r.last_stats.wave_per_walker == 1
