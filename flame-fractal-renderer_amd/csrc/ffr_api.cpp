@@ -66,6 +66,7 @@ struct ffr_renderer {
     // kChainSlots per xform), no final xform, no colour
     bool chain_ok = false;
     bool chain_post_id = false; // ... and every post-affine is the identity
+    bool chain_shared = false;  // ... and one pre-affine matrix and weight
     size_t lds_bytes = 0;
     // buffer
     void *dbuf = nullptr;
@@ -174,6 +175,23 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
         for (int i = 0; i < d.dims; ++i)
             if (!(X[d.lay.postb + i] == (R)0 && !std::signbit(X[d.lay.postb + i])))
                 r->chain_post_id = false;
+    }
+    // every xform applies the same pre-affine matrix (bit for bit, and in
+    // 3-D every one applies it) and one variation of the same weight: the
+    // kernel holds them once and broadcasts only the offsets
+    r->chain_shared = d.slot_type.size() == 1 && d.nxf > 0;
+    for (int x = 0; x < d.nxf && r->chain_shared; ++x) {
+        const R *X = d.table.data() + d.xf0 + (size_t)x * (size_t)d.lay.stride;
+        const R *X0 = d.table.data() + d.xf0;
+        if (std::memcmp(X + d.lay.preA, X0 + d.lay.preA, sizeof(R) * (size_t)(d.dims * d.dims)))
+            r->chain_shared = false;
+        if (d.dims == 3 && X[d.lay.has_pre] == (R)0)
+            r->chain_shared = false;
+        if (!(d.masks[(size_t)x] & 1ull))
+            r->chain_shared = false;
+        const R *w = d.table.data() + d.slot_off[0] + (size_t)x * (size_t)d.slot_stride[0];
+        if (std::memcmp(w, d.table.data() + d.slot_off[0], sizeof(R)))
+            r->chain_shared = false;
     }
     r->nxf = d.nxf;
     r->order = d.order;
@@ -589,6 +607,7 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     L.chain = p.chain;
     L.chain_slots = (int)d.slot_type.size();
     L.chain_post_id = r->chain_post_id;
+    L.chain_shared = r->chain_shared;
     L.cache = p.cache;
     L.grid = p.grid;
     L.block = p.block;

@@ -110,6 +110,7 @@ struct WalkLaunch {
     bool chain = false;               // one flame, one walker per wave (chain_kernel)
     int chain_slots = 0;              // its variation slots (1..kChainSlots)
     bool chain_post_id = false;       // every xform's post-affine is the identity
+    bool chain_shared = false;        // one pre-affine matrix and weight for all
 };
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;

@@ -1,7 +1,9 @@
 """The wave-per-walker kernel (few walkers of a linear flame, no final xform,
 no colour) against the lane-per-walker kernel and the oracle.  Flames with
 identity post-affines take its constant-post build, the others (three_slots,
-negzero_post) read the post-affine from the table.
+negzero_post) read the post-affine from the table; maps that share one
+matrix and weight (sierpinski, tetrahedron) take the build that holds them
+once, the others (uneven_weights, fern, ...) broadcast every coefficient.
 
 A render with the cell cache left to the renderer (-1) and fewer walkers
 than 16 per CU takes the wave-per-walker kernel; `cell_cache=0` forces the
@@ -75,8 +77,20 @@ def negzero_post(size=96):
     return json.dumps(d)
 
 
+def uneven_weights(size=96):
+    """Sierpinski's shared matrix, but the three linear variations carry
+    different weights (and one matrix a -0.0 entry): the kernel broadcasts
+    every xform's coefficients instead of holding one copy"""
+    d = load(sierpinski(size))
+    for k, x in enumerate(d["xforms"]):
+        x["variations"] = [{"name": "linear", "weight": [1.0, 0.9, 1.1][k]}]
+    d["xforms"][2]["pre_affine"]["A"] = [[0.5, -0.0], [0.0, 0.5]]
+    return json.dumps(d)
+
+
 FLAMES = {
     "sierpinski": (sierpinski, 2),
+    "uneven_weights": (uneven_weights, 2),
     "negzero_post": (negzero_post, 2),
     "fern": (TF.fern, 2),
     "tetrahedron": (TF.tetrahedron, 3),
@@ -134,7 +148,8 @@ def test_chain_equals_lane_kernel(ffr_mod, name, mode):
         b.close()
 
 
-@pytest.mark.parametrize("flame", ["three_slots", "negzero_post", "sierpinski"])
+@pytest.mark.parametrize("flame", ["three_slots", "negzero_post", "sierpinski",
+                                   "uneven_weights"])
 @pytest.mark.parametrize("mode", ["fp64", "fp32"])
 def test_chain_matches_oracle(ffr_mod, oracle_mod, mode, flame):
     text = FLAMES[flame][0]()
