@@ -1,5 +1,5 @@
-"""The wave-per-walker kernel (few walkers of a linear flame, no final xform,
-no colour) against the lane-per-walker kernel and the oracle.  Flames with
+"""The wave-per-walker kernel (few walkers of a linear flame without colour,
+final xform or not) against the lane-per-walker kernel and the oracle.  Flames with
 identity post-affines take its constant-post build, the others (three_slots,
 negzero_post) read the post-affine from the table; maps that share one
 matrix and weight (sierpinski, tetrahedron) take the build that holds them
@@ -98,6 +98,7 @@ FLAMES = {
     "one_d": (TF.one_d, 1),
     "tied_weights": (TF.tied_weights, 2),
     "three_slots": (three_slots, 2),
+    "rotations_final": (TF.rotations, 2),
 }
 
 
@@ -149,7 +150,7 @@ def test_chain_equals_lane_kernel(ffr_mod, name, mode):
 
 
 @pytest.mark.parametrize("flame", ["three_slots", "negzero_post", "sierpinski",
-                                   "uneven_weights"])
+                                   "uneven_weights", "rotations_final"])
 @pytest.mark.parametrize("mode", ["fp64", "fp32"])
 def test_chain_matches_oracle(ffr_mod, oracle_mod, mode, flame):
     text = FLAMES[flame][0]()
@@ -168,11 +169,14 @@ def test_chain_matches_oracle(ffr_mod, oracle_mod, mode, flame):
     r.close()
 
 
+@pytest.mark.parametrize("flame", ["no_final", "final"])
 @pytest.mark.parametrize("mode", ["fp64", "fp32u64"])
-def test_chain_bad_values_restart_exact(ffr_mod, oracle_mod, mode):
+def test_chain_bad_values_restart_exact(ffr_mod, oracle_mod, mode, flame):
     """bad values end a block: the walker starts again from the next word;
-    every bad point, its xform, the counts and extremes equal the oracle's"""
-    text = bad_no_final()
+    every bad point, its xform, the counts and extremes equal the oracle's
+    (with a final xform, the plotted point of a bad step is the final
+    xform's image of the bad point)"""
+    text = bad_no_final() if flame == "no_final" else TF.occasional_bad()
     samples, batch = 300_000, 1500
     r, ok = render(ffr_mod, text, samples, batch, mode, -1, bad_value_limit=10 ** 9)
     assert ok and r.last_stats.wave_per_walker == 1
@@ -218,13 +222,10 @@ def test_chain_at_walker_limit_and_cumulative(ffr_mod):
     b.close()
 
 
-@pytest.mark.parametrize("case", ["final", "colour", "cache", "block", "five_slots",
-                                  "sinusoidal"])
+@pytest.mark.parametrize("case", ["colour", "cache", "block", "five_slots", "sinusoidal"])
 def test_chain_not_taken(ffr_mod, case):
     text, kw = sierpinski(64), {}
-    if case == "final":
-        text = TF.rotations()
-    elif case == "colour":
+    if case == "colour":
         d = load(text)
         d["color_dimensions"] = 1
         for x in d["xforms"]:
