@@ -91,7 +91,7 @@ constexpr int kHotSlots = 1 << 15;
 // takes (beyond, the lane-per-walker kernel has enough waves in flight)
 constexpr int kChainSlots = 4;
 constexpr int kChainWaves = 4;
-constexpr int kChainWalkersPerCu = 16;
+constexpr int kChainWalkersPerCu = 32;
 
 // a walker-kernel launch: one flame (args, a device WalkArgs) or a batch
 // (descs, device WalkArgs[] indexed by the tasks' flame numbers)

@@ -5,8 +5,8 @@ negzero_post) read the post-affine from the table; maps that share one
 matrix and weight (sierpinski, tetrahedron) take the build that holds them
 once, the others (uneven_weights, fern, ...) broadcast every coefficient.
 
-A render with the cell cache left to the renderer (-1) and fewer walkers
-than 16 per CU takes the wave-per-walker kernel; `cell_cache=0` forces the
+A render with the cell cache left to the renderer (-1) and at most 32
+walkers per CU takes the wave-per-walker kernel; `cell_cache=0` forces the
 lane-per-walker kernel on the same walkers and streams.  Buffers, samples
 iterated and plotted, xform counts, point extremes and bad values must be
 equal bit for bit, in every precision mode and at walker shapes that leave
@@ -204,12 +204,12 @@ def test_chain_bad_value_limit_stops(ffr_mod):
 
 
 def test_chain_at_walker_limit_and_cumulative(ffr_mod):
-    """the most walkers the kernel takes (16 per CU), then a second render
+    """the most walkers the kernel takes (32 per CU), then a second render
     onto the same buffer: cumulative statistics equal the lane kernel's"""
     text = sierpinski(256)
     a = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=300)
     b = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=300, cell_cache=0)
-    walkers = 16 * ffr_mod.DeviceMemory.cu_count()
+    walkers = 32 * ffr_mod.DeviceMemory.cu_count()
     for rep in range(2):
         print(f"walker limit {walkers}, render {rep}", flush=True)
         assert a.render(walkers * 300) and b.render(walkers * 300)
