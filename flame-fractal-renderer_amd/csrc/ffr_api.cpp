@@ -63,7 +63,7 @@ struct ffr_renderer {
     DevTables tab;
     int vs = 2;
     // the wave-per-walker kernel applies: linear variations only (at most
-    // kChainSlots per xform), no colour
+    // kChainSlots per xform), at most four colour dimensions
     bool chain_ok = false;
     bool chain_post_id = false; // ... and every post-affine is the identity
     bool chain_shared = false;  // ... and one pre-affine matrix and weight
@@ -158,7 +158,7 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
                                            " bytes exceeds the " + std::to_string(max_lds) +
                                            "-byte LDS of a workgroup");
     r->vs = pick_variation_set(d.type_set);
-    r->chain_ok = r->vs == 0 /* linear only, final xform included */ && d.r == 0 &&
+    r->chain_ok = r->vs == 0 /* linear only, final xform included */ && d.r <= 4 &&
                   d.slot_type.size() <= (size_t)kChainSlots;
     // every xform's post-affine is exactly the identity (entries 1.0 and
     // +0.0, bit for bit): the wave-per-walker kernel then applies it with

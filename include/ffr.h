@@ -78,7 +78,8 @@ typedef struct ffr_stats {
                                  cell cache or flushed it (no reference counterpart)  */
     uint64_t flush_atomics;   /* of those, the LDS cell-cache flushes                   */
     uint32_t wave_per_walker; /* 1: the last render ran one walker per wavefront (few
-                                 walkers of a linear flame), 0: one per lane          */
+                                 walkers of a linear flame, up to four colour
+                                 dimensions), 0: one per lane                         */
     uint32_t cache_slots;     /* LDS cell-cache slots per workgroup of the last render
                                  (0: no cache)                                        */
 } ffr_stats;

@@ -1,5 +1,6 @@
-"""The wave-per-walker kernel (few walkers of a linear flame without colour,
-final xform or not) against the lane-per-walker kernel and the oracle.  Flames with
+"""The wave-per-walker kernel (few walkers of a linear flame, with up to four
+colour dimensions, final xform or not) against the lane-per-walker kernel
+and the oracle.  Flames with
 identity post-affines take its constant-post build, the others (three_slots,
 negzero_post) read the post-affine from the table; maps that share one
 matrix and weight (sierpinski, tetrahedron) take the build that holds them
@@ -195,6 +196,36 @@ def test_chain_bad_values_restart_exact(ffr_mod, oracle_mod, mode, flame):
     lane.close()
 
 
+@pytest.mark.parametrize("r_dims,final_color", [(3, True), (3, False), (1, True), (4, True)])
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "fp32u64"])
+def test_chain_colour_equals_lane_kernel(ffr_mod, oracle_mod, r_dims, final_color, mode):
+    """linear colour flames (a final xform with or without its own colour):
+    counts and statistics equal the lane kernel's and the oracle's, colour
+    sums agree to the rounding of their atomic order"""
+    text = TF.color_flame(r_dims, final_color)
+    samples, batch = 400_000, 2000
+    print(f"colour r={r_dims} final_color={final_color} {mode}", flush=True)
+    a, oka = render(ffr_mod, text, samples, batch, mode, -1)
+    b, okb = render(ffr_mod, text, samples, batch, mode, 0)
+    assert oka and okb
+    assert a.last_stats.wave_per_walker == 1 and b.last_stats.wave_per_walker == 0
+    wa, wb = a.read_buffer(), b.read_buffer()
+    assert np.array_equal(a.counts(wa), b.counts(wb))
+    tol = 1e-12 if mode != "fp32" else 2e-5
+    ca, cb = a.colors(wa), b.colors(wb)
+    assert np.allclose(ca, cb, rtol=tol, atol=tol * float(np.abs(cb).max()))
+    assert a.samples_iterated == b.samples_iterated
+    assert a.samples_plotted == b.samples_plotted
+    assert np.array_equal(a.xform_distribution(), b.xform_distribution())
+    assert a.point_extremes == b.point_extremes
+    ref = oracle_mod.OracleFlame(text, MODES[mode], settle=-1).render(
+        samples, batch, seed=SEED, threads=1)
+    assert ref.success and np.array_equal(a.counts(wa), ref.counts)
+    assert np.allclose(ca, ref.colors, rtol=tol, atol=tol * float(np.abs(ref.colors).max()))
+    a.close()
+    b.close()
+
+
 def test_chain_bad_value_limit_stops(ffr_mod):
     text = TF.expanding()
     r, ok = render(ffr_mod, text, 100_000, 1000, "fp64", -1, bad_value_limit=0)
@@ -222,14 +253,14 @@ def test_chain_at_walker_limit_and_cumulative(ffr_mod):
     b.close()
 
 
-@pytest.mark.parametrize("case", ["colour", "cache", "block", "five_slots", "sinusoidal"])
+@pytest.mark.parametrize("case", ["colour5", "cache", "block", "five_slots", "sinusoidal"])
 def test_chain_not_taken(ffr_mod, case):
     text, kw = sierpinski(64), {}
-    if case == "colour":
+    if case == "colour5":
         d = load(text)
-        d["color_dimensions"] = 1
+        d["color_dimensions"] = 5
         for x in d["xforms"]:
-            x["color"] = [0.5]
+            x["color"] = [0.5] * 5
         text = json.dumps(d)
     elif case == "cache":
         kw["cell_cache"] = 1
