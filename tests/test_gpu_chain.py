@@ -226,6 +226,44 @@ def test_chain_colour_equals_lane_kernel(ffr_mod, oracle_mod, r_dims, final_colo
     b.close()
 
 
+def bad_colour(size=64):
+    """bad_no_final with two colour dimensions: a restart draws new colour
+    words, and the bad step's plotted colour is the one before it"""
+    d = json.loads(bad_no_final(size))
+    d["color_dimensions"] = 2
+    for k, x in enumerate(d["xforms"]):
+        x["color"] = [0.2 + 0.3 * k, 0.9 - 0.4 * k]
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_chain_colour_bad_values(ffr_mod, oracle_mod, mode):
+    text = bad_colour()
+    samples, batch = 300_000, 1500
+    a, oka = render(ffr_mod, text, samples, batch, mode, -1, bad_value_limit=10 ** 9)
+    b, okb = render(ffr_mod, text, samples, batch, mode, 0, bad_value_limit=10 ** 9)
+    assert oka and okb
+    assert a.last_stats.wave_per_walker == 1 and b.last_stats.wave_per_walker == 0
+    wa, wb = a.read_buffer(), b.read_buffer()
+    assert np.array_equal(a.counts(wa), b.counts(wb))
+    tol = 1e-12 if mode != "fp32" else 2e-5
+    ca, cb = a.colors(wa), b.colors(wb)
+    assert np.allclose(ca, cb, rtol=tol, atol=tol * float(np.abs(cb).max()))
+    assert a.samples_iterated == b.samples_iterated
+    assert a.samples_plotted == b.samples_plotted
+    assert np.array_equal(a.xform_distribution(), b.xform_distribution())
+    assert a.last_stats.n_bad == b.last_stats.n_bad > 200
+    ia, pa = a.bad_values()
+    ib, pb = b.bad_values()
+    assert records(ia, pa) == records(ib, pb)
+    ref = oracle_mod.OracleFlame(text, MODES[mode], settle=-1).render(
+        samples, batch, seed=SEED, threads=1, bv_limit=10 ** 9, bv_cap=4096)
+    assert ref.success and np.array_equal(a.counts(wa), ref.counts)
+    assert np.allclose(ca, ref.colors, rtol=tol, atol=tol * float(np.abs(ref.colors).max()))
+    a.close()
+    b.close()
+
+
 def test_chain_bad_value_limit_stops(ffr_mod):
     text = TF.expanding()
     r, ok = render(ffr_mod, text, 100_000, 1000, "fp64", -1, bad_value_limit=0)
