@@ -1659,7 +1659,10 @@ static int NM(walker)(const NM(flame) * f, CT *buf, uint64_t seed,
                     for (int i = 0; i < dims; ++i)
                         sh->bv_pts[nb * 3 + i] = (double)p[i];
             }
-            nb = ++sh->n_bad;
+            /* other walkers read the count without the lock (the limit
+             * test at their start): the store is atomic */
+            nb = sh->n_bad + 1;
+            __atomic_store_n(&sh->n_bad, nb, __ATOMIC_RELAXED);
             pthread_mutex_unlock(&sh->mu);
             if (nb > bv_limit)
                 break;
