@@ -478,7 +478,9 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
     // into parallel draws and plots around a short serial part (unless the
     // cache or the workgroup size is forced)
     if (sh.cache < 0 && sh.block <= 0 && base_lds == 0 && r->chain_ok &&
-        walkers <= (uint64_t)r->cu_count * (uint64_t)kChainWalkersPerCu) {
+        walkers <= (uint64_t)r->cu_count *
+                       (uint64_t)chain_walkers_per_cu(r->chain_shared, r->chain_post_id,
+                                                      r->spec.color_dims)) {
         p.chain = true;
         p.block = 64 * kChainWaves;
         p.task_walkers = (uint64_t)kChainWaves;

@@ -88,10 +88,25 @@ constexpr int kHotSlots = 1 << 15;
 
 // the wave-per-walker kernel (chain_kernel.inc): variation slots per xform
 // it holds, walkers (waves) per workgroup, and the most walkers per CU it
-// takes (beyond, the lane-per-walker kernel has enough waves in flight)
+// takes (beyond, the lane-per-walker kernel has enough waves in flight).
+// The limit depends on what its serial step broadcasts.  Measured at 8, 16
+// and 32 walkers of 16384 per CU against the lane kernel: the build that
+// holds one shared matrix and weight with identity post-affines wins at 32
+// (Sierpinski FP64 10.7 ms against 15.8); builds that broadcast every
+// xform's matrix or read post-affines from the table lose at 32 (three
+// linear slots with post-affines: 29.4 against 22.6 ms) and win at 16
+// (15.4 against 22.6); colour builds lose at 16 with post-affines (43.1
+// against 39.3 ms) and win at 8 (22.0 against 26.5).
 constexpr int kChainSlots = 4;
 constexpr int kChainWaves = 4;
-constexpr int kChainWalkersPerCu = 32;
+constexpr int kChainWalkersPerCu = 32;      // shared matrix, identity post, no colour
+constexpr int kChainWalkersPerCuGeneral = 16; // other builds without colour
+constexpr int kChainWalkersPerCuColour = 8;   // colour builds
+__host__ __device__ constexpr int chain_walkers_per_cu(bool shared, bool post_id, int r)
+{
+    return r > 0 ? kChainWalkersPerCuColour
+                 : (shared && post_id) ? kChainWalkersPerCu : kChainWalkersPerCuGeneral;
+}
 
 // a walker-kernel launch: one flame (args, a device WalkArgs) or a batch
 // (descs, device WalkArgs[] indexed by the tasks' flame numbers)

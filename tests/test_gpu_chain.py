@@ -226,6 +226,62 @@ def test_chain_colour_equals_lane_kernel(ffr_mod, oracle_mod, r_dims, final_colo
     b.close()
 
 
+def with_colour(text, r=3, seed=21):
+    """the flame with r colour dimensions: every xform but the last gets a
+    colour, the second its own speed"""
+    d = load(text)
+    rng = np.random.default_rng(seed)
+    d["color_dimensions"] = r
+    d["color_speed"] = 0.45
+    for k, x in enumerate(d["xforms"][:-1]):
+        x["color"] = rng.uniform(0, 1, r).tolist()
+        if k == 1:
+            x["color_speed"] = 0.2
+    return json.dumps(d)
+
+
+# the kernel builds the colour flames above leave out: colour in 3-D (the
+# has_pre / has_post flags beside the colour blend), colour with
+# post-affines read from the table, colour with one shared matrix and weight
+COLOUR_BUILDS = {
+    "tetrahedron_3d": (lambda: with_colour(TF.tetrahedron()), 3),
+    "tetrahedron_post_3d": (lambda: with_colour(TF.tetrahedron(post=True), r=2), 3),
+    "three_slots_post": (lambda: with_colour(three_slots()), 2),
+    "negzero_post": (lambda: with_colour(negzero_post(), r=4), 2),
+    "sierpinski_shared": (lambda: with_colour(sierpinski(96)), 2),
+}
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "fp32u64"])
+@pytest.mark.parametrize("name", sorted(COLOUR_BUILDS))
+def test_chain_colour_builds(ffr_mod, oracle_mod, name, mode):
+    maker, dims = COLOUR_BUILDS[name]
+    text = maker()
+    samples, batch = 300_000, 1500
+    print(f"colour build {name} {mode}", flush=True)
+    a, oka = render(ffr_mod, text, samples, batch, mode, -1, settle=20)
+    b, okb = render(ffr_mod, text, samples, batch, mode, 0, settle=20)
+    assert oka and okb
+    assert a.last_stats.wave_per_walker == 1 and b.last_stats.wave_per_walker == 0
+    wa, wb = a.read_buffer(), b.read_buffer()
+    assert np.array_equal(a.counts(wa), b.counts(wb))
+    tol = 1e-12 if mode != "fp32" else 2e-5
+    ca, cb = a.colors(wa), b.colors(wb)
+    assert np.allclose(ca, cb, rtol=tol, atol=tol * float(np.abs(cb).max()))
+    assert a.samples_iterated == b.samples_iterated
+    assert a.samples_plotted == b.samples_plotted
+    assert np.array_equal(a.xform_distribution(), b.xform_distribution())
+    assert a.point_extremes == b.point_extremes
+    ref = oracle_mod.OracleFlame(text, MODES[mode], settle=20).render(
+        samples, batch, seed=SEED, threads=8)
+    assert ref.success and np.array_equal(a.counts(wa), ref.counts)
+    assert np.allclose(ca, ref.colors, rtol=tol, atol=tol * float(np.abs(ref.colors).max()))
+    assert a.samples_plotted == ref.s_plot
+    assert np.array_equal(a.xform_distribution(), ref.xf_dist)
+    a.close()
+    b.close()
+
+
 def bad_colour(size=64):
     """bad_no_final with two colour dimensions: a restart draws new colour
     words, and the bad step's plotted colour is the one before it"""
@@ -289,6 +345,22 @@ def test_chain_at_walker_limit_and_cumulative(ffr_mod):
     assert a.last_stats.wave_per_walker == 0
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("name,per_cu", [("sierpinski", 32), ("uneven_weights", 16),
+                                         ("three_slots", 16), ("colour", 8)])
+def test_chain_cutoff_per_build(ffr_mod, name, per_cu):
+    """the most walkers per CU the wave-per-walker kernel takes depends on
+    what its serial step broadcasts: 32 for one shared matrix with identity
+    post-affines, 16 for the other builds, 8 with colour"""
+    text = TF.color_flame(3) if name == "colour" else FLAMES[name][0]()
+    cu = ffr_mod.DeviceMemory.cu_count()
+    for walkers, want in ((per_cu * cu, 1), (per_cu * cu + 1, 0)):
+        r = ffr_mod.BufferRenderer(text, seed=1, batch_size=64)
+        assert r.render(walkers * 64)
+        assert r.last_stats.walkers == walkers
+        assert r.last_stats.wave_per_walker == want, (name, walkers)
+        r.close()
 
 
 @pytest.mark.parametrize("case", ["colour5", "cache", "block", "five_slots", "sinusoidal"])
