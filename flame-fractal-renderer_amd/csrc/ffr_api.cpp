@@ -823,6 +823,27 @@ int ffr_add_buffer(ffr_renderer *r, const void *src, size_t bytes)
     return rc;
 }
 
+int ffr_add_buffer_noalloc(ffr_renderer *r, const void *src, size_t bytes)
+{
+    if (!r || !src)
+        return fail(FFR_E_ARG, "ffr_add_buffer_noalloc: null argument");
+    if (bytes != ffr_buffer_bytes(r))
+        return fail(FFR_E_SIZE, "BufferRenderer::addBuffer(): sizes do not match");
+    const unsigned long long cell_bytes = (unsigned long long)r->cell_words * r->word_bytes;
+    if (cell_bytes > kNoAllocScratchBytes)
+        return fail(FFR_E_SIZE, "ffr_add_buffer_noalloc: a cell of " +
+                                    std::to_string(cell_bytes) +
+                                    " bytes exceeds the staging memory of " +
+                                    std::to_string(kNoAllocScratchBytes) + " bytes");
+    HIP_TRY(hipSetDevice(r->device));
+    const int rc = add_cells_staged_noalloc(r->dbuf, src, bytes, (int)r->cell_words,
+                                            (int)r->word_bytes, r->stream);
+    if (rc)
+        return fail(FFR_E_DEVICE, std::string("ffr_add_buffer_noalloc: ") +
+                                      hipGetErrorString((hipError_t)rc));
+    return FFR_OK;
+}
+
 } // extern "C"
 
 namespace {

@@ -28,6 +28,16 @@ int launch_scatter_atomics(void *buf, unsigned long long cells, int cw, int word
 // zero a buffer with a kernel (usable inside stream capture)
 int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s);
 
+// the staging memory of ffr_add_buffer_noalloc: a file-scope device array
+// of this many bytes per GPU (noalloc.hip)
+constexpr unsigned long long kNoAllocScratchBytes = 16ull << 20;
+// addBuffer through that array, in chunks of whole cells of cw words of
+// word_bytes (a cell must fit the array), on the current device; waits for
+// the stream.  Allocates nothing.  Calls that overlap on one GPU share the
+// array and race.
+int add_cells_staged_noalloc(void *dst, const void *src, unsigned long long bytes, int cw,
+                             int word_bytes, hipStream_t s);
+
 // walker launches, one entry per precision mode and dimension count
 #define FFR_DECL_WALK(mode)                             \
     int launch_walk_##mode##_d1(const WalkLaunch &L); \

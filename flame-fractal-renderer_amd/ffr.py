@@ -146,7 +146,7 @@ EXPORTS = [
     "ffr_render", "ffr_render_async", "ffr_sync", "ffr_set_stream",
     "ffr_device_buffer", "ffr_buffer_cells", "ffr_cell_words", "ffr_word_bytes",
     "ffr_buffer_bytes", "ffr_dims", "ffr_color_dims", "ffr_read_buffer",
-    "ffr_add_buffer", "ffr_write_buffer", "ffr_clear_buffer",
+    "ffr_add_buffer", "ffr_add_buffer_noalloc", "ffr_write_buffer", "ffr_clear_buffer",
     "ffr_add_buffer_stream", "ffr_read_buffer_stream",
     "ffr_histogram_stats", "ffr_xform_distribution", "ffr_xform_id_count",
     "ffr_bad_values", "ffr_xform_order", "ffr_auto_batch", "ffr_check_flame",
@@ -196,7 +196,8 @@ def lib():
     L.ffr_dims.argtypes = [P]
     L.ffr_color_dims.restype = C.c_int32
     L.ffr_color_dims.argtypes = [P]
-    for name in ("ffr_read_buffer", "ffr_add_buffer", "ffr_write_buffer"):
+    for name in ("ffr_read_buffer", "ffr_add_buffer", "ffr_add_buffer_noalloc",
+                 "ffr_write_buffer"):
         getattr(L, name).argtypes = [P, P, C.c_size_t]
     L.ffr_clear_buffer.argtypes = [P]
     L.ffr_add_buffer_stream.argtypes = [P, READ_FN, P]
@@ -464,6 +465,12 @@ class BufferRenderer:
     def add_buffer(self, words: np.ndarray):
         w = np.ascontiguousarray(words, dtype=self._dtype())
         _check(lib().ffr_add_buffer(self._h, w.ctypes.data_as(C.c_void_p), w.nbytes))
+
+    def add_buffer_noalloc(self, words: np.ndarray):
+        """add_buffer staged through the library's fixed device array (no
+        device allocation; calls overlapping on one GPU race)"""
+        w = np.ascontiguousarray(words, dtype=self._dtype())
+        _check(lib().ffr_add_buffer_noalloc(self._h, w.ctypes.data_as(C.c_void_p), w.nbytes))
 
     def clear_buffer(self):
         _check(lib().ffr_clear_buffer(self._h))

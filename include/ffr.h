@@ -133,6 +133,20 @@ int ffr_read_buffer(ffr_renderer *r, void *dst, size_t bytes);
 /* addBuffer (:375-452): add a buffer of the same layout cell by cell
  * (counts as integers, colours as floats); bytes must equal ffr_buffer_bytes */
 int ffr_add_buffer(ffr_renderer *r, const void *src, size_t bytes);
+/* ffr_add_buffer without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops that must not hipMalloc, harnesses that forbid it).  It
+ * synchronises the renderer's stream, so it is not for stream capture.
+ * ffr_add_buffer stages the host words through a device buffer
+ * it allocates per call; this one stages them through a library-owned
+ * device array of 16 MiB per GPU, in chunks of whole cells, with the same
+ * add kernel, so the result is the same bit for bit.  Any buffer size is
+ * taken; a single cell wider than 16 MiB is refused (FFR_E_SIZE).  The
+ * array is held from the first call on a GPU until the process ends.  Every
+ * call on one GPU shares it: calls that overlap on one GPU (two host
+ * threads, two renderers on different streams) race and must be serialised
+ * by the caller.  Returns when the words are added. */
+int ffr_add_buffer_noalloc(ffr_renderer *r, const void *src, size_t bytes);
 /* readBuffer (:457-473): overwrite the buffer */
 int ffr_write_buffer(ffr_renderer *r, const void *src, size_t bytes);
 int ffr_clear_buffer(ffr_renderer *r);
