@@ -1,0 +1,264 @@
+// Binned accumulation of count-only buffers: the walker kernel writes one
+// record per plotted sample (the cell index) instead of an atomic add; these
+// kernels sort the records by tile with a counting sort and add each tile's
+// counts from LDS into the buffer with plain loads and stores.
+//
+// The reference adds one to the cell per plotted sample with a relaxed
+// atomic (src/renderers/buffer_renderer.hpp:211-215).  Counts are integers,
+// so any order of adding them gives the same buffer.  On this device
+// scattered global atomics are resolved at the memory side at ~23 G/s
+// whatever their scope; a buffer far beyond the LDS whose samples spread
+// over more cells than an LDS cache holds is bound by that rate.  Records
+// move at HBM bandwidth instead:
+//   walker   writes 4 B per sample (records pre-filled with kEmptyKey)
+//   hist     reads the records, one tile histogram per block of records
+//   colscan  per tile, the running sum of the blocks' counts (the block's
+//            first slot in the tile's region) and the tile's total
+//   tilescan exclusive scan of the tile totals (the tiles' regions)
+//   scatter  re-reads each block, writes every record to its tile's region
+//   accum    one workgroup per tile: counts in LDS, then one add per cell
+// A tile is kBinTileCells consecutive cells (dimension 0 fastest), whose
+// u32 counts fit the LDS of one workgroup.
+#include "util_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+namespace ffr {
+
+namespace {
+
+constexpr int kBinBlock = 256;                       // lanes per workgroup
+constexpr unsigned kBinEmpty = kEmptyKey;
+
+__global__ void __launch_bounds__(kBinBlock)
+    bin_fill_kernel(unsigned *rec, unsigned long long n)
+{
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x)
+        rec[i] = kBinEmpty;
+}
+
+// records each lane loads before it uses them (independent loads in flight)
+constexpr int kBinUnroll = 8;
+
+// hist[b * ntiles + t] = records of block b in tile t
+__global__ void __launch_bounds__(kBinBlock)
+    bin_hist_kernel(const unsigned *rec, unsigned long long n, unsigned long long per_block,
+                    int tile_bits, int ntiles, unsigned *hist)
+{
+    extern __shared__ unsigned h[];
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        h[t] = 0;
+    __syncthreads();
+    const unsigned long long b0 = (unsigned long long)blockIdx.x * per_block;
+    const unsigned long long b1 = b0 + per_block < n ? b0 + per_block : n;
+    constexpr unsigned long long step = (unsigned long long)kBinBlock * kBinUnroll;
+    for (unsigned long long i0 = b0 + threadIdx.x; i0 < b1; i0 += step) {
+        unsigned c[kBinUnroll];
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j) {
+            const unsigned long long i = i0 + (unsigned long long)j * kBinBlock;
+            c[j] = i < b1 ? rec[i] : kBinEmpty;
+        }
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j)
+            if (c[j] != kBinEmpty)
+                atomicAdd(&h[c[j] >> tile_bits], 1u);
+    }
+    __syncthreads();
+    unsigned *row = hist + (size_t)blockIdx.x * (size_t)ntiles;
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        row[t] = h[t];
+}
+
+// column scan: for tile t, hist[b][t] becomes the records of tiles t in
+// blocks before b (exclusive), total[t] the tile's records
+__global__ void __launch_bounds__(kBinBlock)
+    bin_colscan_kernel(unsigned *hist, int nblocks, int ntiles, unsigned *total)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles)
+        return;
+    unsigned run = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += kBinUnroll) {
+        unsigned v[kBinUnroll];
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j)
+            v[j] = b0 + j < nblocks ? hist[(size_t)(b0 + j) * (size_t)ntiles + t] : 0u;
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j)
+            if (b0 + j < nblocks) {
+                hist[(size_t)(b0 + j) * (size_t)ntiles + t] = run;
+                run += v[j];
+            }
+    }
+    total[t] = run;
+}
+
+// exclusive scan of the tile totals into start[0..ntiles] (one workgroup)
+__global__ void __launch_bounds__(1024) bin_tilescan_kernel(const unsigned *total, int ntiles,
+                                                           unsigned *start)
+{
+    __shared__ unsigned part[1024];
+    const int per = (ntiles + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int t0 = threadIdx.x * per;
+    unsigned s = 0;
+    for (int k = 0; k < per; ++k)
+        if (t0 + k < ntiles)
+            s += total[t0 + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the per-thread sums
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        const unsigned v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (int k = 0; k < per; ++k)
+        if (t0 + k < ntiles) {
+            start[t0 + k] = run;
+            run += total[t0 + k];
+        }
+    if (threadIdx.x == blockDim.x - 1)
+        start[ntiles] = part[threadIdx.x];
+}
+
+// each block writes its records to their tiles' regions: the slot of the
+// k-th record of tile t met by block b is start[t] + hist[b][t] + k (the
+// order of records inside a tile's region does not matter)
+__global__ void __launch_bounds__(kBinBlock)
+    bin_scatter_kernel(const unsigned *rec, unsigned long long n, unsigned long long per_block,
+                       int tile_bits, int ntiles, const unsigned *hist, const unsigned *start,
+                       unsigned *sorted)
+{
+    extern __shared__ unsigned cur[];
+    const unsigned *row = hist + (size_t)blockIdx.x * (size_t)ntiles;
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        cur[t] = start[t] + row[t];
+    __syncthreads();
+    const unsigned long long b0 = (unsigned long long)blockIdx.x * per_block;
+    const unsigned long long b1 = b0 + per_block < n ? b0 + per_block : n;
+    constexpr unsigned long long step = (unsigned long long)kBinBlock * kBinUnroll;
+    for (unsigned long long i0 = b0 + threadIdx.x; i0 < b1; i0 += step) {
+        unsigned c[kBinUnroll], slot[kBinUnroll];
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j) {
+            const unsigned long long i = i0 + (unsigned long long)j * kBinBlock;
+            c[j] = i < b1 ? rec[i] : kBinEmpty;
+        }
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j)
+            slot[j] = c[j] != kBinEmpty ? atomicAdd(&cur[c[j] >> tile_bits], 1u) : 0u;
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j)
+            if (c[j] != kBinEmpty)
+                sorted[slot[j]] = c[j];
+    }
+}
+
+// one workgroup per tile: count the tile's records in LDS, then add each
+// cell's count into the buffer (no other workgroup touches the tile)
+template <typename CT>
+__global__ void __launch_bounds__(1024)
+    bin_accum_kernel(const unsigned *sorted, const unsigned *start, int tile_bits,
+                     unsigned long long cells, CT *buf)
+{
+    extern __shared__ unsigned cnt[];
+    const int t = blockIdx.x;
+    const unsigned s0 = start[t], s1 = start[t + 1];
+    if (s0 == s1)
+        return; // no record in this tile (uniform across the workgroup)
+    const unsigned tile = 1u << tile_bits;
+    const unsigned long long c0 = (unsigned long long)t << tile_bits;
+    const unsigned long long left = cells - c0;
+    const unsigned ncell = left < tile ? (unsigned)left : tile;
+    for (unsigned i = threadIdx.x; i < ncell; i += blockDim.x)
+        cnt[i] = 0;
+    __syncthreads();
+    const unsigned mask = tile - 1u;
+    const unsigned step = blockDim.x * (unsigned)kBinUnroll;
+    for (unsigned i0 = s0 + threadIdx.x; i0 < s1; i0 += step) {
+        unsigned c[kBinUnroll];
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j) {
+            const unsigned i = i0 + (unsigned)j * blockDim.x;
+            c[j] = i < s1 ? sorted[i] : kBinEmpty;
+        }
+#pragma unroll
+        for (int j = 0; j < kBinUnroll; ++j)
+            if (c[j] != kBinEmpty)
+                atomicAdd(&cnt[c[j] & mask], 1u);
+    }
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < ncell; i += blockDim.x) {
+        const unsigned v = cnt[i];
+        if (v)
+            buf[c0 + i] += (CT)v;
+    }
+}
+
+int grid_for(unsigned long long n, int block)
+{
+    unsigned long long g = (n + (unsigned long long)block - 1) / (unsigned long long)block;
+    return (int)std::min<unsigned long long>(std::max<unsigned long long>(g, 1), 4096);
+}
+
+} // namespace
+
+size_t bin_scratch_hist_words(unsigned long long records, int ntiles)
+{
+    const unsigned long long blocks = (records + kBinRecordsPerBlock - 1) / kBinRecordsPerBlock;
+    return (size_t)std::max<unsigned long long>(blocks, 1) * (size_t)ntiles;
+}
+
+int launch_bin_fill(unsigned *rec, unsigned long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(bin_fill_kernel, dim3(grid_for(n, kBinBlock)), dim3(kBinBlock), 0, s, rec,
+                       n);
+    return (int)hipGetLastError();
+}
+
+int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,
+                          unsigned long long cells, void *buf, int word_bytes, hipStream_t s)
+{
+    const int ntiles = (int)((cells + (1ull << kBinTileBits) - 1) >> kBinTileBits);
+    if (ntiles > kBinMaxTiles || n_records > 0xFFFFFFFFull)
+        return (int)hipErrorInvalidValue;
+    const unsigned long long per = kBinRecordsPerBlock;
+    const int nblocks = (int)std::max<unsigned long long>(1, (n_records + per - 1) / per);
+    const size_t hist_lds = sizeof(unsigned) * (size_t)ntiles;
+    hipLaunchKernelGGL(bin_hist_kernel, dim3(nblocks), dim3(kBinBlock), hist_lds, s, b.rec,
+                       n_records, per, kBinTileBits, ntiles, b.hist);
+    hipLaunchKernelGGL(bin_colscan_kernel, dim3((ntiles + kBinBlock - 1) / kBinBlock),
+                       dim3(kBinBlock), 0, s, b.hist, nblocks, ntiles, b.total);
+    hipLaunchKernelGGL(bin_tilescan_kernel, dim3(1), dim3(1024), 0, s, b.total, ntiles, b.start);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks), dim3(kBinBlock), hist_lds, s, b.rec,
+                       n_records, per, kBinTileBits, ntiles, b.hist, b.start, b.sorted);
+    const size_t tile_lds = sizeof(unsigned) << kBinTileBits;
+    if (word_bytes == 8)
+        hipLaunchKernelGGL((bin_accum_kernel<unsigned long long>), dim3(ntiles), dim3(1024),
+                           tile_lds, s, b.sorted, b.start, kBinTileBits, cells,
+                           (unsigned long long *)buf);
+    else
+        hipLaunchKernelGGL((bin_accum_kernel<unsigned>), dim3(ntiles), dim3(1024), tile_lds, s,
+                           b.sorted, b.start, kBinTileBits, cells, (unsigned *)buf);
+    return (int)hipGetLastError();
+}
+
+int prepare_bin_kernels()
+{
+    // the accumulate kernel takes a whole tile of counts in LDS (128 KiB)
+    const int tile_lds = (int)(sizeof(unsigned) << kBinTileBits);
+    hipError_t e = hipFuncSetAttribute((const void *)bin_accum_kernel<unsigned long long>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bin_accum_kernel<unsigned>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
+    return (int)e;
+}
+
+} // namespace ffr

@@ -134,6 +134,28 @@ __device__ __forceinline__ double m_copysign(double a, double b) { return copysi
 __device__ __forceinline__ void m_sincos(double a, double *s, double *c) { sincos(a, s, c); }
 __device__ __forceinline__ void m_sincos(float a, float *s, float *c) { sincosf(a, s, c); }
 
+// FP32 trajectories with the default build's 8-byte buffer words (no
+// reference counterpart): the six basic variations and the colour blend
+// compute in float throughout.  In both reference builds a double literal
+// stays double and promotes float operands, as the reference's expressions
+// do (src/types/types.hpp:29 sets num_t; its literals are double).
+template <typename R, typename CT> __host__ __device__ constexpr bool native_f32()
+{
+    return sizeof(R) == 4 && sizeof(CT) == 8;
+}
+// the type a reference double literal takes in an expression
+template <typename R, bool NATIVE> struct LitT { using type = double; };
+template <typename R> struct LitT<R, true> { using type = R; };
+template <typename R, bool NATIVE> using lit_t = typename LitT<R, NATIVE>::type;
+
+// the colour blend (1 - s) c + s col of RenderIterator::iterate
+// (render_iterator.hpp:112-138), in the reference's operand types
+template <typename R, bool NATIVE> __device__ __forceinline__ R colour_blend(R c, R s, R col)
+{
+    using L = lit_t<R, NATIVE>;
+    return (R)(((L)1.0 - s) * c + s * col);
+}
+
 template <typename R> struct Consts;
 template <> struct Consts<double> {
     static constexpr double eps = 1e-20;           // constants.hpp:17-22

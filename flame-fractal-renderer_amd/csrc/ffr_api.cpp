@@ -116,6 +116,18 @@ struct ffr_renderer {
     size_t lds_per_cu = 65536;
     int last_cache = 0;
     bool last_chain = false;
+    // binned accumulation (-1 chosen per render, 0 off, 1 on where allowed);
+    // FFR_BINNED overrides.  Its records, walker states and per-pass
+    // arguments are allocated on the first binned render and kept.
+    int bin_pref = -1;
+    BinScratch bin;
+    size_t bin_hist_words = 0;
+    size_t bin_tiles = 0;
+    void *wstate = nullptr;
+    size_t wstate_bytes = 0;
+    void *dargs_bin = nullptr;
+    size_t dargs_bin_n = 0;
+    bool last_bin = false;
 };
 
 static int pick_variation_set(const uint32_t ts[4])
@@ -211,6 +223,7 @@ void ffr_default_options(ffr_options *opt)
     opt->bad_value_limit = 256;
     opt->bad_value_cap = 4096;
     opt->cell_cache = -1;
+    opt->binned = -1;
 }
 
 const char *ffr_last_error(void) { return g_err.c_str(); }
@@ -273,6 +286,9 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
             r->cache_kb = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("FFR_BLOCK"))
             r->block_pref = std::atoi(e);
+        r->bin_pref = opt.binned < 0 ? -1 : (opt.binned ? 1 : 0);
+        if (const char *e = std::getenv("FFR_BINNED"))
+            r->bin_pref = std::atoi(e);
     }
     int rc = fp32 ? setup_tables(r.get(), r->f32) : setup_tables(r.get(), r->f64);
     if (rc != FFR_OK) {
@@ -345,6 +361,13 @@ void ffr_destroy(ffr_renderer *r)
     (void)hipFree(r->dargs);
     (void)hipFree(r->dimg);
     (void)hipFree(r->dhot);
+    (void)hipFree(r->bin.rec);
+    (void)hipFree(r->bin.sorted);
+    (void)hipFree(r->bin.hist);
+    (void)hipFree(r->bin.total);
+    (void)hipFree(r->bin.start);
+    (void)hipFree(r->wstate);
+    (void)hipFree(r->dargs_bin);
     for (int k = 0; k < 2; ++k) {
         if (r->pin[k])
             (void)hipHostFree(r->pin[k]);
@@ -449,6 +472,7 @@ static LaunchShape renderer_shape(const ffr_renderer *r)
 // launch geometry of one render pass (see plan_launch)
 struct LaunchPlan {
     bool chain = false; // one walker per wave (chain_kernel)
+    bool bin = false;   // binned accumulation (walk_bin_kernel + bin_kernels)
     bool cache = false;
     int block = 256;
     int grid = 1;
@@ -474,13 +498,32 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
 {
     LaunchPlan p;
     const size_t base = std::max(table_lds(r), base_lds);
+    const bool chain = sh.cache < 0 && sh.block <= 0 && base_lds == 0 && r->chain_ok &&
+                       walkers <= (uint64_t)r->cu_count *
+                                      (uint64_t)chain_walkers_per_cu(
+                                          r->chain_shared, r->chain_post_id, r->spec.color_dims);
+    // binned accumulation: single renders of count-only buffers under 2^28
+    // cells whose walkers all fit one pass's records; chosen from 2^22 cells
+    // on when the lane-per-walker kernel runs (the samples spread beyond
+    // what an LDS cache holds), or when asked for
+    const uint64_t tw_bin = (uint64_t)kBinWalkBlock;
+    const bool bin_ok = base_lds == 0 && r->spec.color_dims == 0 && r->cells < (1ull << 28) &&
+                        (walkers + tw_bin - 1) / tw_bin * tw_bin <= kBinPassRecords;
+    if (bin_ok && (r->bin_pref == 1 ||
+                   (r->bin_pref < 0 && !chain && sh.cache < 0 && sh.block <= 0 &&
+                    r->cells >= (1ull << 22)))) {
+        p.bin = true;
+        p.block = kBinWalkBlock;
+        p.task_walkers = tw_bin;
+        p.grid = (int)std::min<uint64_t>((walkers + tw_bin - 1) / tw_bin, 1u << 30);
+        p.lds = base;
+        (void)batch;
+        return p;
+    }
     // few walkers of a linear flame: one walker per wave, its chain split
     // into parallel draws and plots around a short serial part (unless the
     // cache or the workgroup size is forced)
-    if (sh.cache < 0 && sh.block <= 0 && base_lds == 0 && r->chain_ok &&
-        walkers <= (uint64_t)r->cu_count *
-                       (uint64_t)chain_walkers_per_cu(r->chain_shared, r->chain_post_id,
-                                                      r->spec.color_dims)) {
+    if (chain) {
         p.chain = true;
         p.block = 64 * kChainWaves;
         p.task_walkers = (uint64_t)kChainWaves;
@@ -628,6 +671,111 @@ static void split_samples(const ffr_renderer *r, uint64_t samples, uint64_t &bat
     last = samples - (walkers - 1) * batch;
 }
 
+// grow a device allocation to at least `bytes` (contents not kept)
+static hipError_t ensure_device(void **p, size_t &have, size_t bytes)
+{
+    if (have >= bytes)
+        return hipSuccess;
+    (void)hipFree(*p);
+    *p = nullptr;
+    have = 0;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess)
+        have = bytes;
+    return e;
+}
+
+// Binned accumulation of one render: passes of `span` iterations of every
+// walker (the walkers' points and stream positions carry over in wstate),
+// each pass writing at most kBinPassRecords records, sorted by tile and
+// added into the buffer before the next pass overwrites them.
+template <typename R>
+static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
+                         uint64_t walker0, uint64_t walkers, uint64_t batch, uint64_t last)
+{
+    const uint64_t tw = p.task_walkers;
+    const uint64_t slots = (walkers + tw - 1) / tw * tw; // walker slots of the tasks
+    // FFR_BIN_PASS_RECORDS makes passes smaller (tests run many passes)
+    uint64_t pass_records = kBinPassRecords;
+    if (const char *env = std::getenv("FFR_BIN_PASS_RECORDS")) {
+        const unsigned long long v = std::strtoull(env, nullptr, 10);
+        if (v >= 1 && v < pass_records)
+            pass_records = v;
+    }
+    const uint64_t span = std::max<uint64_t>(1, std::min<uint64_t>(batch, pass_records / slots));
+    const uint64_t npass = (batch + span - 1) / span;
+    const uint64_t cap = slots * span;
+    const size_t ntiles = (size_t)((r->cells + (1ull << kBinTileBits) - 1) >> kBinTileBits);
+    hipError_t e = hipSuccess;
+    size_t have_rec = (size_t)r->bin.capacity * 4, have_sorted = have_rec;
+    if (have_rec < cap * 4) {
+        e = ensure_device((void **)&r->bin.rec, have_rec, (size_t)cap * 4);
+        if (e == hipSuccess)
+            e = ensure_device((void **)&r->bin.sorted, have_sorted, (size_t)cap * 4);
+        r->bin.capacity = e == hipSuccess ? cap : 0;
+    }
+    size_t hist_bytes = r->bin_hist_words * 4;
+    if (e == hipSuccess)
+        e = ensure_device((void **)&r->bin.hist, hist_bytes,
+                          bin_scratch_hist_words(cap, (int)ntiles) * 4);
+    r->bin_hist_words = hist_bytes / 4;
+    size_t tile_bytes = r->bin_tiles * 4;
+    if (e == hipSuccess && tile_bytes < (ntiles + 1) * 4) {
+        size_t t2 = tile_bytes;
+        e = ensure_device((void **)&r->bin.total, tile_bytes, (ntiles + 1) * 4);
+        if (e == hipSuccess)
+            e = ensure_device((void **)&r->bin.start, t2, (ntiles + 1) * 4);
+    }
+    r->bin_tiles = e == hipSuccess ? std::max(r->bin_tiles, ntiles + 1) : 0;
+    if (e == hipSuccess && npass > 1)
+        e = ensure_device(&r->wstate, r->wstate_bytes, (size_t)slots * sizeof(WalkerState<R>));
+    size_t args_bytes = r->dargs_bin_n * sizeof(WalkArgs<R>);
+    if (e == hipSuccess)
+        e = ensure_device(&r->dargs_bin, args_bytes, (size_t)npass * sizeof(WalkArgs<R>));
+    r->dargs_bin_n = args_bytes / sizeof(WalkArgs<R>);
+    if (e != hipSuccess)
+        return (int)e;
+    // one argument block per pass, copied at once (the host copy stays
+    // alive until the copy ran)
+    r->hargs.resize((size_t)npass * sizeof(WalkArgs<R>));
+    for (uint64_t k = 0; k < npass; ++k) {
+        WalkArgs<R> a;
+        walk_args(r, d, p, walker0, walkers, batch, last, a);
+        a.rec = r->bin.rec;
+        a.wstate = r->wstate;
+        a.it0 = k * span;
+        a.it1 = std::min<uint64_t>(batch, a.it0 + span);
+        std::memcpy(r->hargs.data() + k * sizeof a, &a, sizeof a);
+    }
+    e = hipMemcpyAsync(r->dargs_bin, r->hargs.data(), r->hargs.size(), hipMemcpyHostToDevice,
+                       r->stream);
+    if (e != hipSuccess)
+        return (int)e;
+    for (uint64_t k = 0; k < npass; ++k) {
+        const uint64_t n_rec = slots * (std::min<uint64_t>(batch, (k + 1) * span) - k * span);
+        int rc = launch_bin_fill(r->bin.rec, n_rec, r->stream);
+        if (rc)
+            return rc;
+        WalkLaunch L;
+        L.nc = 0;
+        L.vs = r->vs;
+        L.bin = true;
+        L.grid = p.grid;
+        L.block = p.block;
+        L.lds = p.lds;
+        L.stream = r->stream;
+        L.args = (const unsigned char *)r->dargs_bin + k * sizeof(WalkArgs<R>);
+        rc = launch_walk_any(r->mode, r->spec.dims, L);
+        if (rc)
+            return rc;
+        rc = launch_bin_accumulate(r->bin, n_rec, r->cells, r->dbuf, (int)r->word_bytes,
+                                   r->stream);
+        if (rc)
+            return rc;
+    }
+    return 0;
+}
+
 static int launch(ffr_renderer *r, uint64_t samples)
 {
     uint64_t batch, walkers, last;
@@ -637,13 +785,27 @@ static int launch(ffr_renderer *r, uint64_t samples)
     r->last_batch = batch;
     r->last_cache = p.cache ? p.slots : 0;
     r->last_chain = p.chain;
+    r->last_bin = p.bin;
     int rc = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream);
     if (rc)
         return fail(FFR_E_DEVICE, std::string("stats reset: ") + hipGetErrorString((hipError_t)rc));
+    if (p.bin) {
+        static std::once_flag prepared;
+        static int prep_rc = 0;
+        std::call_once(prepared, [] { prep_rc = prepare_bin_kernels(); });
+        if (prep_rc)
+            return fail(FFR_E_DEVICE, std::string("binned accumulation: ") +
+                                          hipGetErrorString((hipError_t)prep_rc));
+    }
     HIP_TRY(hipEventRecord(r->ev0, r->stream));
-    rc = r->mode == MODE_F64_U64
-             ? launch_single(r, r->f64, p, r->next_walker, walkers, batch, last)
-             : launch_single(r, r->f32, p, r->next_walker, walkers, batch, last);
+    if (p.bin)
+        rc = r->mode == MODE_F64_U64
+                 ? launch_binned(r, r->f64, p, r->next_walker, walkers, batch, last)
+                 : launch_binned(r, r->f32, p, r->next_walker, walkers, batch, last);
+    else
+        rc = r->mode == MODE_F64_U64
+                 ? launch_single(r, r->f64, p, r->next_walker, walkers, batch, last)
+                 : launch_single(r, r->f32, p, r->next_walker, walkers, batch, last);
     if (rc)
         return fail(FFR_E_DEVICE, std::string("walker kernel launch: ") +
                                       hipGetErrorString((hipError_t)rc));
@@ -708,6 +870,7 @@ static int finish(ffr_renderer *r, ffr_stats *out)
         out->flush_atomics = r->last_flush_atomics;
         out->wave_per_walker = r->last_chain ? 1u : 0u;
         out->cache_slots = (uint32_t)r->last_cache;
+        out->binned = r->last_bin ? 1u : 0u;
     }
     if (r->last_bad > r->opt.bad_value_limit)
         return fail(FFR_E_BADVALUE_LIMIT, "render failure (flame may not be sufficiently contractive)");

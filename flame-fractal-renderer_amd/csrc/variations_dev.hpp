@@ -1,12 +1,14 @@
 // Device variation functions (reference: src/variations/variations.hpp).
 //
-// var_eval<R, D, VS>(type, E, t, out, st) evaluates variation `type` on the
+// var_eval<R, D, VS, NATIVE>(type, E, t, out, st) evaluates variation `type` on the
 // pre-affine point t.  E points at the variation's table entry:
 // [weight, (axis_x, axis_y when a 2-D variation runs in 3-D), params...]
 // where params are the values the reference constructor precomputes
 // (computed on the host in num_t, flame_model.cpp var_setup).  Expressions
 // keep the reference's operand types: double literals promote float
-// operands to double exactly as in the reference's float build.
+// operands to double exactly as in the reference's float build (NATIVE:
+// FP32 trajectories with 8-byte buffer words compute the six basic
+// variations in float, see native_f32).
 // The caller multiplies by the weight and sums; random words are drawn from
 // the walker stream in the reference's order.
 //
@@ -88,9 +90,10 @@ template <typename R> __device__ __forceinline__ R dev_norm2sq(const R *x, int n
 // correctly rounded float division gives the same bits, at float cost.
 template <typename R> __device__ __forceinline__ R recip_ref(R x) { return (R)1 / x; }
 
-template <typename R>
+template <typename R, bool NATIVE>
 __device__ __forceinline__ void basic6(unsigned need, const R *t, R (&o)[6][2])
 {
+    using L = lit_t<R, NATIVE>;
     const R EPS = Consts<R>::eps;
     const R x = t[0], y = t[1];
     const R n2sq = x * x + y * y;
@@ -117,21 +120,22 @@ __device__ __forceinline__ void basic6(unsigned need, const R *t, R (&o)[6][2])
     if (need & 16u) {
         R r = recip_ref(n2 + EPS);
         R a = (x - y) * (x + y);
-        R b = 2.0 * x * y;
+        R b = (L)2.0 * x * y;
         o[4][0] = a * r;
         o[4][1] = b * r;
     }
     if (need & 32u) {
         R a = m_atan2(y, x);
-        o[5][0] = a * M_1_PI;
-        o[5][1] = n2 - 1.0;
+        o[5][0] = a * (L)M_1_PI;
+        o[5][1] = n2 - (L)1.0;
     }
 }
 
 // 2-D variations (variations.hpp:505-2376)
-template <typename R, int VS>
+template <typename R, int VS, bool NATIVE>
 __device__ __forceinline__ void calc2d(int type, const R *p, R x, R y, R &ox, R &oy, Stream &st)
 {
+    using L = lit_t<R, NATIVE>;
     const R EPS = Consts<R>::eps;
     const R n2sq = x * x + y * y;
 #define N2 ((R)m_sqrt(n2sq))
@@ -146,14 +150,14 @@ __device__ __forceinline__ void calc2d(int type, const R *p, R x, R y, R &ox, R 
     case 21: if constexpr (vs_has(VS, 21)) { // horseshoe
         R r = recip_ref(N2 + EPS);
         R a = (x - y) * (x + y);
-        R b = 2.0 * x * y;
+        R b = (L)2.0 * x * y;
         ox = a * r;
         oy = b * r;
     } break;
     case 22: if constexpr (vs_has(VS, 22)) { // polar
         R a = ANG, r = N2;
-        ox = a * M_1_PI;
-        oy = r - 1.0;
+        ox = a * (L)M_1_PI;
+        oy = r - (L)1.0;
     } break;
     case 23: if constexpr (vs_has(VS, 23)) { // polar2
         ox = ANG;
@@ -777,7 +781,7 @@ __device__ __forceinline__ void calc2d(int type, const R *p, R x, R y, R &ox, R 
 
 // N-D variations and the VariationFrom2D embedding (variations.hpp:63-107,
 // 160-505, 2376-2386).  out must start at zero.
-template <typename R, int D, int VS>
+template <typename R, int D, int VS, bool NATIVE>
 __device__ __forceinline__ void var_eval(int type, const R *E, const R *t, R *out, Stream &st)
 {
     const R EPS = Consts<R>::eps;
@@ -785,7 +789,7 @@ __device__ __forceinline__ void var_eval(int type, const R *E, const R *t, R *ou
         if constexpr (D >= 2) {
             if constexpr (D == 2) {
                 R ox = 0, oy = 0;
-                calc2d<R, VS>(type, E + 1, t[0], t[1], ox, oy, st);
+                calc2d<R, VS, NATIVE>(type, E + 1, t[0], t[1], ox, oy, st);
                 out[0] = ox;
                 out[1] = oy;
             } else {
@@ -793,7 +797,7 @@ __device__ __forceinline__ void var_eval(int type, const R *E, const R *t, R *ou
                 R xx = ax == 0 ? t[0] : (ax == 1 ? t[1] : t[2]);
                 R yy = ay == 0 ? t[0] : (ay == 1 ? t[1] : t[2]);
                 R ox = 0, oy = 0;
-                calc2d<R, VS>(type, E + 3, xx, yy, ox, oy, st);
+                calc2d<R, VS, NATIVE>(type, E + 3, xx, yy, ox, oy, st);
 #pragma unroll
                 for (int i = 0; i < D; ++i)
                     out[i] = (i == ax) ? ox : ((i == ay) ? oy : (R)0);

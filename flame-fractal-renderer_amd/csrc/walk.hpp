@@ -66,7 +66,24 @@ template <typename R> struct WalkArgs {
     // xform record layout (flame_model.hpp XformLayout)
     int l_preA, l_preb, l_postA, l_postb, l_speed, l_color, l_has_color, l_has_pre,
         l_has_post, l_stride;
+    // binned accumulation (walk_bin_kernel): this pass runs iterations
+    // [it0, it1) of every walker and writes each plotted sample's cell
+    // index to rec (task-major, then iteration, then lane); walkers keep
+    // their point and stream position in wstate between passes
+    unsigned *rec;
+    void *wstate;
+    uint64_t it0, it1;
 };
+
+// lanes per workgroup (= walkers per task) of the binned walker kernel
+constexpr int kBinWalkBlock = 256;
+
+// a walker between two passes of binned accumulation
+template <typename R> struct WalkerState {
+    R p[3];
+    uint64_t n; // stream position, or kStoppedWalker
+};
+constexpr uint64_t kStoppedWalker = ~0ull;
 
 // one renderer's statistics, for the reset of a batched launch
 struct ResetEntry {
@@ -126,6 +143,7 @@ struct WalkLaunch {
     int chain_slots = 0;              // its variation slots (1..kChainSlots)
     bool chain_post_id = false;       // every xform's post-affine is the identity
     bool chain_shared = false;        // one pre-affine matrix and weight for all
+    bool bin = false;                 // binned accumulation (walk_bin_kernel)
 };
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;

@@ -44,7 +44,7 @@ class Options(C.Structure):
         ("cell_cache", C.c_int32),
         ("cache_kib", C.c_int32),
         ("block", C.c_int32),
-        ("reserved", C.c_int32),
+        ("binned", C.c_int32),
     ]
 
 
@@ -62,6 +62,8 @@ class Stats(C.Structure):
         ("flush_atomics", C.c_uint64),
         ("wave_per_walker", C.c_uint32),
         ("cache_slots", C.c_uint32),
+        ("binned", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -351,7 +353,8 @@ class BufferRenderer:
     def __init__(self, flame_json: str, precision: str = "fp64", word: int = 64,
                  seed: int = 0, batch_size: int = 0, bad_value_limit: int = 256,
                  settle: int = -1, device: int = 0, bad_value_cap: int = 4096,
-                 cell_cache: int = -1, cache_kib: int = 0, block: int = 0):
+                 cell_cache: int = -1, cache_kib: int = 0, block: int = 0,
+                 binned: int = -1):
         L = lib()
         o = Options()
         L.ffr_default_options(C.byref(o))
@@ -366,6 +369,7 @@ class BufferRenderer:
         o.cell_cache = cell_cache
         o.cache_kib = cache_kib
         o.block = block
+        o.binned = binned
         h = C.c_void_p()
         b = flame_json.encode()
         _check(L.ffr_create(b, len(b), C.byref(o), C.byref(h)))

@@ -61,7 +61,14 @@ typedef struct ffr_options {
     int32_t cell_cache;
     int32_t cache_kib;
     int32_t block;
-    int32_t reserved;
+    /* binned accumulation of count-only buffers: every plotted sample's cell
+     * goes to a record in device memory instead of an atomic add, the
+     * records are sorted by tile of 32768 cells and each tile's counts are
+     * added from LDS (-1 chosen per render: on for flames without colour
+     * whose buffer has at least 2^22 and under 2^28 cells when the
+     * lane-per-walker kernel runs; 0 off; 1 on where the flame allows it:
+     * no colour, under 2^28 cells).  The counts are the same either way. */
+    int32_t binned;
 } ffr_options;
 
 typedef struct ffr_stats {
@@ -82,10 +89,12 @@ typedef struct ffr_stats {
                                  dimensions), 0: one per lane                         */
     uint32_t cache_slots;     /* LDS cell-cache slots per workgroup of the last render
                                  (0: no cache)                                        */
+    uint32_t binned;          /* 1: the last render used binned accumulation          */
+    uint32_t reserved;
 } ffr_stats;
 
 /* defaults: FP64, WORD64, settle -1, device 0, seed 0, batch 0, limit 256,
- * cell_cache -1, cache_kib 0, block 0 */
+ * cell_cache -1, cache_kib 0, block 0, binned -1 */
 void ffr_default_options(ffr_options *opt);
 
 /*

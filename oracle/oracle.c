@@ -468,36 +468,42 @@ void orc_sort_order(const double *weights, int32_t n, int32_t mode,
 #define CR double
 #define SFX f64u64
 #define R_IS_FLOAT 0
+#define NATIVE_F32 0
 #include "oracle_impl.h"
 #undef R
 #undef CT
 #undef CR
 #undef SFX
 #undef R_IS_FLOAT
+#undef NATIVE_F32
 
 #define R float
 #define CT uint32_t
 #define CR float
 #define SFX f32u32
 #define R_IS_FLOAT 1
+#define NATIVE_F32 0
 #include "oracle_impl.h"
 #undef R
 #undef CT
 #undef CR
 #undef SFX
 #undef R_IS_FLOAT
+#undef NATIVE_F32
 
 #define R float
 #define CT uint64_t
 #define CR double
 #define SFX f32u64
 #define R_IS_FLOAT 1
+#define NATIVE_F32 1
 #include "oracle_impl.h"
 #undef R
 #undef CT
 #undef CR
 #undef SFX
 #undef R_IS_FLOAT
+#undef NATIVE_F32
 
 /* ------------------------------------------------------------------ */
 /* mode dispatch                                                        */

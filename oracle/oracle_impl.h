@@ -458,6 +458,19 @@ static int NM(var_setup)(NM(var) * v, const orc_var *raw, int dims, char *err,
 
 /* ---------------- 2-D variations (variations.hpp:505-2376) ---------------- */
 
+/* LIT(v): a double literal of the reference.  In both reference builds it
+ * stays double and promotes float operands (src/types/types.hpp:29 sets
+ * num_t; the literals are double).  FP32 trajectories with the default
+ * build's 8-byte buffer words have no reference counterpart: there the six
+ * basic variations and the colour blend compute in float throughout, and
+ * LIT rounds the literal to num_t. */
+#undef LIT
+#if NATIVE_F32
+#define LIT(v) ((R)(v))
+#else
+#define LIT(v) (v)
+#endif
+
 static void NM(calc2d)(const NM(var) * v, R x, R y, R *ox, R *oy,
                        orc_stream *s)
 {
@@ -478,15 +491,15 @@ static void NM(calc2d)(const NM(var) * v, R x, R y, R *ox, R *oy,
     case 21: { /* horseshoe */
         R r = 1.0 / (N2 + EPS);
         R a = (x - y) * (x + y);
-        R b = 2.0 * x * y;
+        R b = LIT(2.0) * x * y;
         *ox = a * r;
         *oy = b * r;
         return;
     }
     case 22: { /* polar */
         R a = ANG, r = N2;
-        *ox = a * M_1_PI;
-        *oy = r - 1.0;
+        *ox = a * LIT(M_1_PI);
+        *oy = r - LIT(1.0);
         return;
     }
     case 23: /* polar2 */
@@ -1622,7 +1635,7 @@ static int NM(walker)(const NM(flame) * f, CT *buf, uint64_t seed,
         NM(xf_apply)(xf, dims, p, &st);
         if (xf->has_color)
             for (int i = 0; i < r; ++i)
-                c[i] = (1.0 - sp) * c[i] + sp * xf->color[i];
+                c[i] = (LIT(1.0) - sp) * c[i] + sp * xf->color[i];
         if (f->has_final) {
             const NM(xf) *ff = &f->fin;
             sp = ff->speed;
@@ -1631,7 +1644,7 @@ static int NM(walker)(const NM(flame) * f, CT *buf, uint64_t seed,
             NM(xf_apply)(ff, dims, pf, &st);
             if (ff->has_color)
                 for (int i = 0; i < r; ++i)
-                    cf[i] = (1.0 - sp) * c[i] + sp * ff->color[i];
+                    cf[i] = (LIT(1.0) - sp) * c[i] + sp * ff->color[i];
             else
                 for (int i = 0; i < r; ++i)
                     cf[i] = c[i];

@@ -37,7 +37,7 @@ def test_default_options(ffr_mod):
     o = ffr_mod.Options()
     ffr_mod.lib().ffr_default_options(C.byref(o))
     assert (o.precision, o.word, o.settle_iters, o.bad_value_limit) == (0, 0, -1, 256)
-    assert (o.cell_cache, o.cache_kib, o.block) == (-1, 0, 0)
+    assert (o.cell_cache, o.cache_kib, o.block, o.binned) == (-1, 0, 0, -1)
     # the header's struct: 4 x 32-bit, 4 x 64-bit, 4 x 32-bit fields
     assert C.sizeof(o) == 64
 
