@@ -207,6 +207,7 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
         "kernel_ms": kavg,
         "walkers": int(r.last_stats.walkers),
         "batch": int(r.last_stats.batch_size),
+        "binned": bool(r.last_stats.binned),
         "plotted_per_step": plotted,
         "roofline": {
             "bound": "hbm",
@@ -313,6 +314,52 @@ def run_batch(ffr, texts, samples, precision, keep, seed0=C4_SEED):
     for r in rs:
         r.close()
     return out, hists
+
+
+# per-flame check of the batch's FP32 renders against the FP64 oracle: a
+# reduced sample count per flame, the same walker streams and settle count
+C4_ORACLE_SAMPLES = 1 << 20
+
+
+def batch_vs_oracle(ffr, texts):
+    """Every batch flame rendered with FP32 trajectories on the GPU (one
+    batched pass) and with FP64 on the CPU oracle, C4_ORACLE_SAMPLES each, on
+    the same streams: per-flame L1 of the normalised histograms, next to the
+    L1 between two FP64 oracle renders on different streams (the sampling
+    floor at this count)."""
+    import numpy as np
+    from oracle import oracle
+
+    n = C4_ORACLE_SAMPLES
+    batch = int(ffr.lib().ffr_auto_batch(None, n))
+    cores = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+             else os.cpu_count())
+    rs = [ffr.BufferRenderer(t, precision="fp32", seed=C4_SEED + i, batch_size=batch,
+                             settle=SETTLE_FP64) for i, t in enumerate(texts)]
+    with ffr.Batch(rs) as b:
+        if not all(b.render([n] * len(rs))):
+            raise RuntimeError("a batched flame stopped at the bad-value limit")
+    d, floor = [], []
+    t0 = time.perf_counter()
+    for i, (t, r) in enumerate(zip(texts, rs)):
+        g = r.counts().astype(np.float64)
+        r.close()
+        f = oracle.OracleFlame(t, oracle.F64_U64, settle=SETTLE_FP64)
+        a = f.render(n, batch, seed=C4_SEED + i, threads=cores).counts.astype(np.float64)
+        o = f.render(n, batch, seed=C4_SEED + 0x10000 + i, threads=cores).counts.astype(
+            np.float64)
+        d.append(float(np.abs(g / g.sum() - a / a.sum()).sum()))
+        floor.append(float(np.abs(o / o.sum() - a / a.sum()).sum()))
+    cpu_s = time.perf_counter() - t0
+    return {
+        "samples_per_flame": n,
+        "l1_fp32_gpu_vs_fp64_oracle": {"mean": sum(d) / len(d), "max": max(d),
+                                       "over_1e-3": int(sum(1 for x in d if x > 1e-3))},
+        "l1_fp64_oracle_other_streams": {"mean": sum(floor) / len(floor),
+                                         "min": min(floor)},
+        "per_flame": [round(x, 7) for x in d],
+        "oracle_s": cpu_s, "oracle_threads": cores,
+    }
 
 
 def serving_trace(texts2d, cells2d):
@@ -501,6 +548,12 @@ def main():
         log(f"bench c3: {C3_SAMPLES} samples/step")
         res, _ = run_gpu(ffr, text, precision="fp32", word=32, samples=C3_SAMPLES, steps=S,
                          warmup=W, bytes_per_plot=4, seed=C3_SEED)
+        if res["binned"]:
+            # binned accumulation moves records, not atomics: per sample 4 B
+            # written by the walker, the sentinel fill, the tile histogram's
+            # read, the scatter's read and write, the accumulation's read
+            res["record_bytes_per_sample"] = 24
+            res["record_traffic_GBs"] = (24 * C3_SAMPLES / (res["kernel_ms"] * 1e-3) / 1e9)
         workloads["tetrahedron_512_fp32"] = res
         log("tetrahedron_512_fp32", json.dumps(res))
 
@@ -544,6 +597,12 @@ def main():
             "probe_l1_max_fp32_flames": max([probe[i] for i in range(len(texts))
                                              if policy[i] == "fp32"] or [0.0]),
         }
+        if not args.no_cpu:
+            # the FP32 renders against the FP64 oracle (not the GPU's FP64)
+            res32["vs_fp64_oracle"] = batch_vs_oracle(ffr, texts)
+            log("batch_256_fp32 vs the FP64 oracle",
+                json.dumps({k: v for k, v in res32["vs_fp64_oracle"].items()
+                            if k != "per_flame"}))
         workloads["batch_256_fp32"] = res32
         workloads["batch_256_fp64"] = res64
         workloads["batch_256_policy"] = resp
