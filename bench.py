@@ -208,6 +208,7 @@ def run_gpu(ffr, text, *, precision, samples, steps, warmup, bytes_per_plot, see
         "walkers": int(r.last_stats.walkers),
         "batch": int(r.last_stats.batch_size),
         "binned": bool(r.last_stats.binned),
+        "compiled": bool(r.last_stats.compiled),
         "plotted_per_step": plotted,
         "roofline": {
             "bound": "hbm",

@@ -2,8 +2,30 @@
 // random stream and precision-overloaded math.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
+#else
+// compiled at run time (a flame's own kernel): the runtime header is
+// implicit, the <math.h> constants are not
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#define M_PI_2 1.57079632679489661923
+#define M_PI_4 0.78539816339744830962
+#define M_1_PI 0.31830988618379067154
+#define M_2_PI 0.63661977236758134308
+#endif
+#ifndef INFINITY
+#define INFINITY __builtin_huge_valf()
+#endif
+// the run-time compiler declares the fixed-width integers in its own
+// namespace
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#endif
 
 namespace ffr {
 

@@ -3,6 +3,7 @@
 #include "../../include/ffr.h"
 
 #include "flame_model.hpp"
+#include "rtc.hpp"
 #include "util_kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -128,6 +129,17 @@ struct ffr_renderer {
     void *dargs_bin = nullptr;
     size_t dargs_bin_n = 0;
     bool last_bin = false;
+    // the flame compiled into its own kernel (-1 chosen by a probe, 0 off,
+    // 1 on); FFR_COMPILED overrides.  rtc_fn: the kernel for the coming
+    // launch (nullptr: the table-driven one); compiled_choice: the probe's
+    // verdict per launch geometry (cache on/off, lanes per workgroup)
+    int compiled_pref = -1;
+    void *rtc_fn = nullptr;
+    bool last_compiled = false;
+    std::unordered_map<int, bool> compiled_choice;
+    // renders launched and samples asked for, over the renderer's life
+    uint64_t launches = 0;
+    uint64_t samples_asked = 0;
 };
 
 static int pick_variation_set(const uint32_t ts[4])
@@ -224,6 +236,7 @@ void ffr_default_options(ffr_options *opt)
     opt->bad_value_cap = 4096;
     opt->cell_cache = -1;
     opt->binned = -1;
+    opt->compiled = -1;
 }
 
 const char *ffr_last_error(void) { return g_err.c_str(); }
@@ -289,6 +302,9 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
         r->bin_pref = opt.binned < 0 ? -1 : (opt.binned ? 1 : 0);
         if (const char *e = std::getenv("FFR_BINNED"))
             r->bin_pref = std::atoi(e);
+        r->compiled_pref = opt.compiled < 0 ? -1 : (opt.compiled ? 1 : 0);
+        if (const char *e = std::getenv("FFR_COMPILED"))
+            r->compiled_pref = std::atoi(e) < 0 ? -1 : (std::atoi(e) ? 1 : 0);
     }
     int rc = fp32 ? setup_tables(r.get(), r->f32) : setup_tables(r.get(), r->f64);
     if (rc != FFR_OK) {
@@ -446,6 +462,10 @@ static void fill_args(ffr_renderer *r, const DeviceFlame<R> &d, WalkArgs<R> &a)
     a.bt0 = d.bt0;
     a.bt_stride = d.bt_stride;
     a.bt_nvmax = d.bt_nvmax;
+    a.bt_union = 0;
+    if (d.bt0 >= 0)
+        for (int x = 0; x < d.nxf; ++x)
+            a.bt_union |= (int)d.table[(size_t)(d.bt0 + x * d.bt_stride + 1)];
     for (size_t j = 0; j < d.fin_type.size(); ++j) {
         a.fin_type[j] = d.fin_type[j];
         a.fin_off[j] = d.fin_off[j];
@@ -634,6 +654,152 @@ static int launch_walk_any(int mode, int dims, const WalkLaunch &L)
     }
 }
 
+// the colour-register class the walker kernel is built for (launch_walk)
+static int nc_class(int r, bool cache)
+{
+    if (cache && r <= 4)
+        return r == 0 ? 0 : 4;
+    return r == 0 ? 0 : (r <= 4 ? 4 : 128);
+}
+
+static int launch_walk_any(int mode, int dims, const WalkLaunch &L);
+
+// the table-driven walker kernel for a plan, its arguments at r->dargs
+static int launch_table_kernel(ffr_renderer *r, const LaunchPlan &p, int nslots)
+{
+    WalkLaunch L;
+    L.nc = r->spec.color_dims;
+    L.vs = r->vs;
+    L.chain = p.chain;
+    L.chain_slots = nslots;
+    L.chain_post_id = r->chain_post_id;
+    L.chain_shared = r->chain_shared;
+    L.cache = p.cache;
+    L.grid = p.grid;
+    L.block = p.block;
+    L.lds = p.lds;
+    L.stream = r->stream;
+    L.args = r->dargs;
+    return launch_walk_any(r->mode, r->spec.dims, L);
+}
+
+// the flame's compiled kernel, its arguments (the WalkDyn part) at r->dargs
+static int launch_compiled_kernel(ffr_renderer *r, const LaunchPlan &p, void *fn)
+{
+    void *dargs = r->dargs;
+    void *params[] = {&dargs};
+    return (int)hipModuleLaunchKernel((hipFunction_t)fn, (unsigned)p.grid, 1, 1,
+                                      (unsigned)p.block, 1, 1, (unsigned)p.lds, r->stream,
+                                      params, nullptr);
+}
+
+// With compiled -1 a renderer in steady use (its second render or later,
+// once its renders have asked for this many samples in all) compiles the
+// flame's kernel and probes whether it beats the table-driven one: the
+// compile (about half a second) pays off only over repeated renders.
+constexpr uint64_t kCompiledProbeSamples = 1ull << 33;
+
+// Time both kernels on the launch geometry of the coming render, with
+// walkers of at most 1024 samples, into a scratch buffer: the renderer's
+// buffer and stream positions are untouched, and the statistics the probe
+// leaves are reset before the render.  *faster: the compiled kernel took
+// less time (the better of two runs each).
+template <typename R>
+static int probe_compiled(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
+                          uint64_t walker0, uint64_t walkers, uint64_t batch, void *fn,
+                          bool *faster)
+{
+    const uint64_t pb = std::min<uint64_t>(batch, 1024);
+    void *scratch = nullptr;
+    HIP_TRY(hipMalloc(&scratch, ffr_buffer_bytes(r)));
+    WalkArgs<R> a;
+    walk_args(r, d, p, walker0, walkers, pb, pb, a);
+    a.buf = scratch;
+    a.bv_limit = ~0ull; // the probe never stops at the bad-value limit
+    std::vector<unsigned char> h(sizeof a);
+    std::memcpy(h.data(), &a, sizeof a);
+    hipError_t e = hipMemcpyAsync(r->dargs, h.data(), sizeof a, hipMemcpyHostToDevice, r->stream);
+    float best[2] = {1e30f, 1e30f};
+    for (int round = 0; round < 2 && e == hipSuccess; ++round)
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream) ? hipErrorLaunchFailure
+                                                                        : hipSuccess;
+            if (e == hipSuccess)
+                e = hipEventRecord(r->ev0, r->stream);
+            if (e == hipSuccess)
+                e = (hipError_t)(k ? launch_compiled_kernel(r, p, fn)
+                                   : launch_table_kernel(r, p, (int)d.slot_type.size()));
+            if (e == hipSuccess)
+                e = hipEventRecord(r->ev1, r->stream);
+            if (e == hipSuccess)
+                e = hipEventSynchronize(r->ev1);
+            float ms = 0;
+            if (e == hipSuccess)
+                e = hipEventElapsedTime(&ms, r->ev0, r->ev1);
+            if (e == hipSuccess)
+                best[k] = std::min(best[k], ms);
+        }
+    const hipError_t ef = hipFree(scratch);
+    if (e == hipSuccess)
+        e = ef;
+    if (e != hipSuccess)
+        return fail(FFR_E_DEVICE, std::string("compiled-kernel probe: ") + hipGetErrorString(e));
+    *faster = best[1] < best[0];
+    return FFR_OK;
+}
+
+// the flame's compiled kernel for this plan (lane-per-walker kernel only),
+// compiled on first use; r->rtc_fn stays nullptr when the plan takes
+// another kernel, compiled kernels are off, or the probe chose the table
+template <typename R>
+static int prepare_compiled(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
+                            uint64_t walker0, uint64_t walkers, uint64_t batch, uint64_t last,
+                            uint64_t samples)
+{
+    r->rtc_fn = nullptr;
+    if (!r->compiled_pref || p.chain || p.bin)
+        return FFR_OK;
+    const int key = (p.cache ? 1 : 0) | (p.block << 1);
+    const auto known = r->compiled_choice.find(key);
+    const bool probe = r->compiled_pref < 0 && known == r->compiled_choice.end();
+    if (r->compiled_pref < 0) {
+        if (known != r->compiled_choice.end() && !known->second)
+            return FFR_OK;
+        // FFR_COMPILED_PROBE_SAMPLES lowers the threshold (tests probe small
+        // renders)
+        uint64_t threshold = kCompiledProbeSamples;
+        if (const char *env = std::getenv("FFR_COMPILED_PROBE_SAMPLES"))
+            threshold = std::strtoull(env, nullptr, 10);
+        if (probe && (r->launches == 0 || r->samples_asked + samples < threshold))
+            return FFR_OK;
+    }
+    WalkArgs<R> a;
+    walk_args(r, d, p, walker0, walkers, batch, last, a);
+    RtcKernel k;
+    k.mode = r->mode;
+    k.dims = r->spec.dims;
+    k.nc = nc_class(r->spec.color_dims, p.cache);
+    k.vs = r->vs;
+    k.cache = p.cache;
+    k.block_max = p.cache ? cache_block_max(r->mode == MODE_F64_U64, r->vs) : 256;
+    std::string err;
+    double compile_s = 0;
+    void *fn = nullptr;
+    if (rtc_flame_kernel(a, d.cw, k, r->device, &fn, &compile_s, err) != 0)
+        return fail(FFR_E_DEVICE, err);
+    if (probe) {
+        bool faster = false;
+        const int rc = probe_compiled(r, d, p, walker0, walkers, batch, fn, &faster);
+        if (rc != FFR_OK)
+            return rc;
+        r->compiled_choice[key] = faster;
+        if (!faster)
+            return FFR_OK;
+    }
+    r->rtc_fn = fn;
+    return FFR_OK;
+}
+
 template <typename R>
 static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
                          uint64_t walker0, uint64_t walkers, uint64_t batch, uint64_t last)
@@ -642,10 +808,15 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     walk_args(r, d, p, walker0, walkers, batch, last, a);
     r->hargs.resize(sizeof a);
     std::memcpy(r->hargs.data(), &a, sizeof a);
-    hipError_t e = hipMemcpyAsync(r->dargs, r->hargs.data(), sizeof a, hipMemcpyHostToDevice,
+    // a compiled flame's kernel reads only the launch part (WalkDyn, at the
+    // start of WalkArgs); its flame fields are constants
+    const size_t nbytes = r->rtc_fn ? sizeof(WalkDyn<R>) : sizeof a;
+    hipError_t e = hipMemcpyAsync(r->dargs, r->hargs.data(), nbytes, hipMemcpyHostToDevice,
                                   r->stream);
     if (e != hipSuccess)
         return (int)e;
+    if (r->rtc_fn)
+        return launch_compiled_kernel(r, p, r->rtc_fn);
     WalkLaunch L;
     L.nc = r->spec.color_dims;
     L.vs = r->vs;
@@ -786,7 +957,15 @@ static int launch(ffr_renderer *r, uint64_t samples)
     r->last_cache = p.cache ? p.slots : 0;
     r->last_chain = p.chain;
     r->last_bin = p.bin;
-    int rc = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream);
+    // (the compiled-kernel probe runs first: the reset after it clears what
+    // the probe counted)
+    int rc = r->mode == MODE_F64_U64
+                 ? prepare_compiled(r, r->f64, p, r->next_walker, walkers, batch, last, samples)
+                 : prepare_compiled(r, r->f32, p, r->next_walker, walkers, batch, last, samples);
+    if (rc != FFR_OK)
+        return rc;
+    r->last_compiled = r->rtc_fn != nullptr;
+    rc = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream);
     if (rc)
         return fail(FFR_E_DEVICE, std::string("stats reset: ") + hipGetErrorString((hipError_t)rc));
     if (p.bin) {
@@ -811,6 +990,8 @@ static int launch(ffr_renderer *r, uint64_t samples)
                                       hipGetErrorString((hipError_t)rc));
     HIP_TRY(hipEventRecord(r->ev1, r->stream));
     r->next_walker += walkers;
+    r->launches += 1;
+    r->samples_asked += samples;
     r->pending = true;
     r->timed = true;
     return FFR_OK;
@@ -871,6 +1052,7 @@ static int finish(ffr_renderer *r, ffr_stats *out)
         out->wave_per_walker = r->last_chain ? 1u : 0u;
         out->cache_slots = (uint32_t)r->last_cache;
         out->binned = r->last_bin ? 1u : 0u;
+        out->compiled = r->last_compiled ? 1u : 0u;
     }
     if (r->last_bad > r->opt.bad_value_limit)
         return fail(FFR_E_BADVALUE_LIMIT, "render failure (flame may not be sufficiently contractive)");

@@ -1,8 +1,10 @@
 // Walker kernel arguments and launch entry (host and device).
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace ffr {
 
@@ -21,7 +23,15 @@ constexpr int kMaxSlots = 64;     // variation slots per flame
 constexpr int kMaxFinalVars = 16; // variations of the final xform
 constexpr int kMaxXforms = 1024;  // xforms after removing weight 0
 
-template <typename R> struct WalkArgs {
+// The walker kernel's arguments in two parts.  WalkDyn holds what changes
+// from launch to launch: device addresses, seed and walker positions, the
+// bad-value limits, the LDS cell cache's geometry and the binned passes.
+// WalkArgs adds what the flame fixes: bounds and cell geometry, the xform
+// count, the table layout and the variation slots.  A flame compiled into
+// its own kernel (rtc.cpp) holds the second part as compile-time constants
+// of a type derived from WalkDyn; the kernel templates read both parts
+// through `a.field` either way.
+template <typename R> struct WalkDyn {
     const R *table;            // flame table (see DeviceFlame), copied to LDS
     const unsigned long long *masks; // [nxf] used-slot masks, copied to LDS
     void *buf;                 // cells * (1 + r) words
@@ -33,9 +43,6 @@ template <typename R> struct WalkArgs {
     // key of slot 1, ...]; each task seeds its cache from them and leaves
     // its own hot cells for the next (nullptr: no hints)
     unsigned *hot;
-    R lo[3], hi[3], mult_d[3];
-    uint64_t mult_i[3];
-    uint64_t cells;
     uint64_t seed;
     uint64_t walker_base;      // stream index of walker 0 of this launch
     uint64_t n_walkers;
@@ -44,9 +51,7 @@ template <typename R> struct WalkArgs {
     uint64_t bv_limit;
     uint64_t bv_cap;
     uint64_t task_walkers;     // walkers per workgroup task (one flame's launch)
-    int nxf, n_slots, has_final, fin_nv, settle, r;
-    int tab_reals;
-    int xf0, fin0;
+    int settle;
     // LDS cell cache (kernels built with it): cache_slots entries of
     // [key u32 | count u32 | r colour words], probed cache_ways deep; when
     // cache_direct the cell index is the slot (the whole buffer fits)
@@ -54,18 +59,6 @@ template <typename R> struct WalkArgs {
     // hints apply when hot[0] == hot_tag (the same cache geometry); a slot
     // is left as a hint when its task counted at least hot_thresh there
     unsigned hot_tag, hot_thresh;
-    // slot and final-variation descriptors live in the kernel arguments so
-    // the kernel reads them with scalar loads
-    int slot_type[kMaxSlots];
-    int slot_off[kMaxSlots];    // table offset of the slot's entries
-    int slot_stride[kMaxSlots]; // entry size
-    int fin_type[kMaxFinalVars];
-    int fin_off[kMaxFinalVars];
-    // per-xform variation lists of all-basic 2-D flames (DeviceFlame::bt0)
-    int bt0, bt_stride, bt_nvmax, bt_pad;
-    // xform record layout (flame_model.hpp XformLayout)
-    int l_preA, l_preb, l_postA, l_postb, l_speed, l_color, l_has_color, l_has_pre,
-        l_has_post, l_stride;
     // binned accumulation (walk_bin_kernel): this pass runs iterations
     // [it0, it1) of every walker and writes each plotted sample's cell
     // index to rec (task-major, then iteration, then lane); walkers keep
@@ -73,6 +66,31 @@ template <typename R> struct WalkArgs {
     unsigned *rec;
     void *wstate;
     uint64_t it0, it1;
+};
+
+template <typename R> struct WalkArgs : WalkDyn<R> {
+    // the flame fields below are run-time values (a compiled flame's type
+    // sets this true and holds them, and cw[], as constants)
+    static constexpr bool kCompiled = false;
+    R lo[3], hi[3], mult_d[3];
+    uint64_t mult_i[3];
+    uint64_t cells;
+    int nxf, n_slots, has_final, fin_nv, r;
+    int tab_reals;
+    int xf0, fin0;
+    // slot and final-variation descriptors live in the kernel arguments so
+    // the kernel reads them with scalar loads
+    int slot_type[kMaxSlots];
+    int slot_off[kMaxSlots];    // table offset of the slot's entries
+    int slot_stride[kMaxSlots]; // entry size
+    int fin_type[kMaxFinalVars];
+    int fin_off[kMaxFinalVars];
+    // per-xform variation lists of all-basic 2-D flames (DeviceFlame::bt0);
+    // bt_union: bit k set when some xform's list holds basic code k
+    int bt0, bt_stride, bt_nvmax, bt_union;
+    // xform record layout (flame_model.hpp XformLayout)
+    int l_preA, l_preb, l_postA, l_postb, l_speed, l_color, l_has_color, l_has_pre,
+        l_has_post, l_stride;
 };
 
 // lanes per workgroup (= walkers per task) of the binned walker kernel
@@ -125,6 +143,7 @@ __host__ __device__ constexpr int chain_walkers_per_cu(bool shared, bool post_id
                  : (shared && post_id) ? kChainWalkersPerCu : kChainWalkersPerCuGeneral;
 }
 
+#ifndef __HIPCC_RTC__
 // a walker-kernel launch: one flame (args, a device WalkArgs) or a batch
 // (descs, device WalkArgs[] indexed by the tasks' flame numbers)
 struct WalkLaunch {
@@ -145,6 +164,7 @@ struct WalkLaunch {
     bool chain_shared = false;        // one pre-affine matrix and weight for all
     bool bin = false;                 // binned accumulation (walk_bin_kernel)
 };
+#endif // __HIPCC_RTC__
 
 constexpr unsigned kEmptyKey = 0xFFFFFFFFu;
 

@@ -45,6 +45,8 @@ class Options(C.Structure):
         ("cache_kib", C.c_int32),
         ("block", C.c_int32),
         ("binned", C.c_int32),
+        ("compiled", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -63,7 +65,7 @@ class Stats(C.Structure):
         ("wave_per_walker", C.c_uint32),
         ("cache_slots", C.c_uint32),
         ("binned", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("compiled", C.c_uint32),
     ]
 
 
@@ -354,7 +356,7 @@ class BufferRenderer:
                  seed: int = 0, batch_size: int = 0, bad_value_limit: int = 256,
                  settle: int = -1, device: int = 0, bad_value_cap: int = 4096,
                  cell_cache: int = -1, cache_kib: int = 0, block: int = 0,
-                 binned: int = -1):
+                 binned: int = -1, compiled: int = -1):
         L = lib()
         o = Options()
         L.ffr_default_options(C.byref(o))
@@ -370,6 +372,7 @@ class BufferRenderer:
         o.cache_kib = cache_kib
         o.block = block
         o.binned = binned
+        o.compiled = compiled
         h = C.c_void_p()
         b = flame_json.encode()
         _check(L.ffr_create(b, len(b), C.byref(o), C.byref(h)))

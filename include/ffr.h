@@ -69,6 +69,16 @@ typedef struct ffr_options {
      * lane-per-walker kernel runs; 0 off; 1 on where the flame allows it:
      * no colour, under 2^28 cells).  The counts are the same either way. */
     int32_t binned;
+    /* the flame compiled into its own kernel (no counterpart in the
+     * reference, whose flame is objects with virtual calls): the walker code
+     * built at run time with hiprtc, the flame's xform count, table layout,
+     * variation types and bounds as constants, kept for the process, for
+     * renders that take the lane-per-walker kernel (-1, the default: chosen
+     * by a timed probe of both kernels, on a scratch buffer, once a renderer
+     * renders again after its renders have asked for 2^33 samples in all;
+     * 0 off; 1 on).  Results equal the table-driven kernel's bit for bit. */
+    int32_t compiled;
+    int32_t reserved;
 } ffr_options;
 
 typedef struct ffr_stats {
@@ -90,11 +100,11 @@ typedef struct ffr_stats {
     uint32_t cache_slots;     /* LDS cell-cache slots per workgroup of the last render
                                  (0: no cache)                                        */
     uint32_t binned;          /* 1: the last render used binned accumulation          */
-    uint32_t reserved;
+    uint32_t compiled;        /* 1: the last render ran the flame's compiled kernel    */
 } ffr_stats;
 
 /* defaults: FP64, WORD64, settle -1, device 0, seed 0, batch 0, limit 256,
- * cell_cache -1, cache_kib 0, block 0, binned -1 */
+ * cell_cache -1, cache_kib 0, block 0, binned -1, compiled -1 */
 void ffr_default_options(ffr_options *opt);
 
 /*

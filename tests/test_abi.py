@@ -37,9 +37,9 @@ def test_default_options(ffr_mod):
     o = ffr_mod.Options()
     ffr_mod.lib().ffr_default_options(C.byref(o))
     assert (o.precision, o.word, o.settle_iters, o.bad_value_limit) == (0, 0, -1, 256)
-    assert (o.cell_cache, o.cache_kib, o.block, o.binned) == (-1, 0, 0, -1)
-    # the header's struct: 4 x 32-bit, 4 x 64-bit, 4 x 32-bit fields
-    assert C.sizeof(o) == 64
+    assert (o.cell_cache, o.cache_kib, o.block, o.binned, o.compiled) == (-1, 0, 0, -1, -1)
+    # the header's struct: 4 x 32-bit, 4 x 64-bit, 6 x 32-bit fields
+    assert C.sizeof(o) == 72
 
 
 def test_check_flame_without_device(ffr_mod):
