@@ -483,6 +483,11 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run kernels of known traffic (counter scaling)")
     ap.add_argument("--fp32-only", action="store_true")
+    ap.add_argument("--float-build", action="store_true",
+                    help="config 2 in FP32 with 4-byte words only: the reference's float "
+                         "build (float trajectories, FP64 literals promote, u32 counts and "
+                         "f32 colour sums), for counter comparisons with the native FP32 "
+                         "trajectories")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the renderer has no multi-GPU path "
@@ -500,6 +505,11 @@ def main():
 
     fp32 = fp64 = None
     l1_c2 = floor_c2 = None
+    if args.float_build:
+        res, _ = run_gpu(ffr, c2_text, precision="fp32", word=32, samples=C2_SAMPLES, steps=S,
+                         warmup=W, bytes_per_plot=16, seed=C2_SEED, settle=SETTLE_FP64)
+        log("c2 float build", json.dumps(res))
+        return 0
     if want("c2"):
         log(f"bench c2: {C2_SAMPLES} samples/step, warmup {W}, steps {S}")
         fp32, h32 = run_gpu(ffr, c2_text, precision="fp32", samples=C2_SAMPLES, steps=S,
@@ -536,6 +546,11 @@ def main():
                           bytes_per_plot=8, seed=C1_SEED + 0x100, batch=65536, settle=20,
                           keep=True)
         workloads["sierpinski_1024_fp32"]["l1_fp64_vs_fp64_other_seed"] = l1(h64b, h64)
+        # what the precision policy picks for this render (its probe renders
+        # 1/64 of the samples in both precisions on the same streams)
+        choice, probe_l1 = ffr.choose_precision(text, C1_SAMPLES, seed=C1_SEED, batch_size=65536,
+                                                settle=20)
+        workloads["sierpinski_1024_fp32"]["policy"] = {"choice": choice, "probe_l1": probe_l1}
         if not args.no_cpu:
             rate, n, dt, cores = cpu_rate(text, oracle.F64_U64, 0, C1_SEED, 65536, settle=20,
                                           fixed=C1_SAMPLES)

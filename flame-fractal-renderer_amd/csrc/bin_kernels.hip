@@ -15,7 +15,9 @@
 //   colscan  per tile, the running sum of the blocks' counts (the block's
 //            first slot in the tile's region) and the tile's total
 //   tilescan exclusive scan of the tile totals (the tiles' regions)
-//   scatter  re-reads each block, writes every record to its tile's region
+//   scatter  re-reads each block in chunks, sorts a chunk by tile in LDS and
+//            writes each tile's run of it to the tile's region (consecutive
+//            lanes, consecutive addresses)
 //   accum    one workgroup per tile: counts in LDS, then one add per cell
 // A tile is kBinTileCells consecutive cells (dimension 0 fastest), whose
 // u32 counts fit the LDS of one workgroup.
@@ -127,36 +129,109 @@ __global__ void __launch_bounds__(1024) bin_tilescan_kernel(const unsigned *tota
         start[ntiles] = part[threadIdx.x];
 }
 
-// each block writes its records to their tiles' regions: the slot of the
-// k-th record of tile t met by block b is start[t] + hist[b][t] + k (the
-// order of records inside a tile's region does not matter)
-__global__ void __launch_bounds__(kBinBlock)
+// Each block writes its records to their tiles' regions: the k-th record of
+// tile t that block b writes goes to slot start[t] + hist[b][t] + k (the
+// order inside a tile's region does not matter).  Written one by one from
+// lanes holding records of many tiles, the 4-byte stores land on as many
+// lines and reach HBM as partial writes (five times the record bytes were
+// measured).  So the block takes its records in chunks of kScatterLanes x
+// PER: a counting sort in LDS puts each chunk in tile order, and lane i then
+// writes the chunk's i-th record, so that a tile's run of the chunk goes out
+// from consecutive lanes to consecutive slots.
+constexpr int kScatterLanes = 1024;
+
+// LDS bytes of the scatter kernel for a tile count: the tiles' next slots,
+// the chunk's counts per tile (offsets after the scan) and the chunk itself
+// (whose first sixteen words hold the wave sums of the scan before it)
+constexpr size_t scatter_lds(int ntiles, int per)
+{
+    return sizeof(unsigned) * (2 * (size_t)ntiles + (size_t)kScatterLanes * (size_t)per);
+}
+
+template <int PER>
+__global__ void __launch_bounds__(kScatterLanes)
     bin_scatter_kernel(const unsigned *rec, unsigned long long n, unsigned long long per_block,
                        int tile_bits, int ntiles, const unsigned *hist, const unsigned *start,
                        unsigned *sorted)
 {
-    extern __shared__ unsigned cur[];
+    extern __shared__ unsigned lds[];
+    unsigned *cur = lds;                   // [ntiles] next slot of each tile
+    unsigned *off = cur + ntiles;          // [ntiles] chunk counts, then offsets
+    unsigned *stage = off + ntiles;        // [kScatterLanes * PER] the chunk in tile order
+    unsigned *wsum = stage;                // [16] wave sums of the scan (before the chunk)
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const unsigned *row = hist + (size_t)blockIdx.x * (size_t)ntiles;
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+    for (int t = tid; t < ntiles; t += kScatterLanes)
         cur[t] = start[t] + row[t];
-    __syncthreads();
     const unsigned long long b0 = (unsigned long long)blockIdx.x * per_block;
     const unsigned long long b1 = b0 + per_block < n ? b0 + per_block : n;
-    constexpr unsigned long long step = (unsigned long long)kBinBlock * kBinUnroll;
-    for (unsigned long long i0 = b0 + threadIdx.x; i0 < b1; i0 += step) {
-        unsigned c[kBinUnroll], slot[kBinUnroll];
+    constexpr unsigned long long chunk = (unsigned long long)kScatterLanes * PER;
+    // tiles each lane scans (consecutive), at most 8 for 8192 tiles
+    const int per_t = (ntiles + kScatterLanes - 1) / kScatterLanes;
+    for (unsigned long long c0 = b0; c0 < b1; c0 += chunk) {
+        for (int t = tid; t < ntiles; t += kScatterLanes)
+            off[t] = 0;
+        __syncthreads();
+        // the chunk's records, counted per tile
+        unsigned c[PER];
 #pragma unroll
-        for (int j = 0; j < kBinUnroll; ++j) {
-            const unsigned long long i = i0 + (unsigned long long)j * kBinBlock;
-            c[j] = i < b1 ? rec[i] : kBinEmpty;
+        for (int k = 0; k < PER; ++k) {
+            const unsigned long long i = c0 + (unsigned long long)k * kScatterLanes + tid;
+            c[k] = i < b1 ? rec[i] : kBinEmpty;
         }
 #pragma unroll
-        for (int j = 0; j < kBinUnroll; ++j)
-            slot[j] = c[j] != kBinEmpty ? atomicAdd(&cur[c[j] >> tile_bits], 1u) : 0u;
+        for (int k = 0; k < PER; ++k)
+            if (c[k] != kBinEmpty)
+                atomicAdd(&off[c[k] >> tile_bits], 1u);
+        __syncthreads();
+        // exclusive scan of the chunk's tile counts (per lane, then per
+        // wave, then across the sixteen waves)
+        const int t0 = tid * per_t;
+        unsigned mine = 0;
+        for (int j = 0; j < per_t; ++j)
+            if (t0 + j < ntiles)
+                mine += off[t0 + j];
+        unsigned incl = mine;
 #pragma unroll
-        for (int j = 0; j < kBinUnroll; ++j)
-            if (c[j] != kBinEmpty)
-                sorted[slot[j]] = c[j];
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(incl, d, 64);
+            if (lane >= d)
+                incl += y;
+        }
+        if (lane == 63)
+            wsum[wave] = incl;
+        __syncthreads();
+        unsigned before = 0, total = 0;
+        for (int w = 0; w < kScatterLanes / 64; ++w) {
+            const unsigned ws = wsum[w];
+            before += w < wave ? ws : 0u;
+            total += ws; // the chunk's records
+        }
+        unsigned run = before + incl - mine;
+        for (int j = 0; j < per_t; ++j)
+            if (t0 + j < ntiles) {
+                const unsigned v = off[t0 + j];
+                off[t0 + j] = run;
+                run += v;
+            }
+        __syncthreads(); // (the wave sums are read before the chunk overwrites them)
+        // each record takes the next place of its tile's run; off[t] then
+        // ends tile t's run, which begins where tile t - 1's ends
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (c[k] != kBinEmpty)
+                stage[atomicAdd(&off[c[k] >> tile_bits], 1u)] = c[k];
+        __syncthreads();
+        for (unsigned i = (unsigned)tid; i < total; i += kScatterLanes) {
+            const unsigned r = stage[i];
+            const unsigned t = r >> tile_bits;
+            sorted[cur[t] + (i - (t ? off[t - 1] : 0u))] = r;
+        }
+        __syncthreads();
+        for (int t = tid; t < ntiles; t += kScatterLanes)
+            cur[t] += off[t] - (t ? off[t - 1] : 0u);
+        __syncthreads();
     }
 }
 
@@ -236,8 +311,14 @@ int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,
     hipLaunchKernelGGL(bin_colscan_kernel, dim3((ntiles + kBinBlock - 1) / kBinBlock),
                        dim3(kBinBlock), 0, s, b.hist, nblocks, ntiles, b.total);
     hipLaunchKernelGGL(bin_tilescan_kernel, dim3(1), dim3(1024), 0, s, b.total, ntiles, b.start);
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(nblocks), dim3(kBinBlock), hist_lds, s, b.rec,
-                       n_records, per, kBinTileBits, ntiles, b.hist, b.start, b.sorted);
+    if (ntiles <= 4096)
+        hipLaunchKernelGGL(bin_scatter_kernel<32>, dim3(nblocks), dim3(kScatterLanes),
+                           scatter_lds(ntiles, 32), s, b.rec, n_records, per, kBinTileBits, ntiles,
+                           b.hist, b.start, b.sorted);
+    else
+        hipLaunchKernelGGL(bin_scatter_kernel<16>, dim3(nblocks), dim3(kScatterLanes),
+                           scatter_lds(ntiles, 16), s, b.rec, n_records, per, kBinTileBits, ntiles,
+                           b.hist, b.start, b.sorted);
     const size_t tile_lds = sizeof(unsigned) << kBinTileBits;
     if (word_bytes == 8)
         hipLaunchKernelGGL((bin_accum_kernel<unsigned long long>), dim3(ntiles), dim3(1024),
@@ -258,6 +339,18 @@ int prepare_bin_kernels()
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void *)bin_accum_kernel<unsigned>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds);
+    // the scatter kernel: chunks of 32 records a lane up to 4096 tiles, of
+    // 16 up to 8192
+    static_assert(scatter_lds(4096, 32) <= 160 * 1024, "scatter chunk beyond the LDS");
+    static_assert(scatter_lds(kBinMaxTiles, 16) <= 160 * 1024, "scatter chunk beyond the LDS");
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bin_scatter_kernel<32>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)scatter_lds(4096, 32));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bin_scatter_kernel<16>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)scatter_lds(kBinMaxTiles, 16));
     return (int)e;
 }
 

@@ -68,6 +68,7 @@ struct ffr_renderer {
     bool chain_ok = false;
     bool chain_post_id = false; // ... and every post-affine is the identity
     bool chain_shared = false;  // ... and one pre-affine matrix and weight
+    size_t chain_lds = 0;       // its LDS bytes
     size_t lds_bytes = 0;
     // buffer
     void *dbuf = nullptr;
@@ -137,6 +138,8 @@ struct ffr_renderer {
     void *rtc_fn = nullptr;
     bool last_compiled = false;
     std::unordered_map<int, bool> compiled_choice;
+    // cell_cache -1: the probe's verdict per walker count (true: cache kept)
+    std::unordered_map<uint64_t, bool> cache_choice;
     // renders launched and samples asked for, over the renderer's life
     uint64_t launches = 0;
     uint64_t samples_asked = 0;
@@ -185,8 +188,8 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
     r->chain_ok = r->vs == 0 /* linear only, final xform included */ && d.r <= 4 &&
                   d.slot_type.size() <= (size_t)kChainSlots;
     // every xform's post-affine is exactly the identity (entries 1.0 and
-    // +0.0, bit for bit): the wave-per-walker kernel then applies it with
-    // constants, the same operations on the same values
+    // +0.0, bit for bit): the wave-per-walker kernel then leaves it out (it
+    // changes a finite point only in the signs of zeros)
     r->chain_post_id = true;
     for (int x = 0; x < d.nxf && r->chain_post_id; ++x) {
         const R *X = d.table.data() + d.xf0 + (size_t)x * (size_t)d.lay.stride;
@@ -217,6 +220,17 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
         if (std::memcmp(w, d.table.data() + d.slot_off[0], sizeof(R)))
             r->chain_shared = false;
     }
+    // its LDS: the table, then kChainRows step rows per wave of the flame's
+    // build
+    const int slots = (int)d.slot_type.size();
+    r->chain_lds =
+        chain_stage_offset(r->tab.tab_reals, (int)sizeof(R), d.nxf) +
+        (size_t)kChainWaves * (size_t)kChainRows * sizeof(R) *
+            (size_t)chain_row_reals((int)sizeof(R), d.dims, d.r > 0 ? 4 : 0, chain_build_ns(slots),
+                                    r->chain_post_id,
+                                    chain_build_sh(r->chain_shared, r->chain_post_id, slots));
+    if (r->chain_lds > (size_t)max_lds)
+        r->chain_ok = false;
     r->nxf = d.nxf;
     r->order = d.order;
     return FFR_OK;
@@ -548,7 +562,7 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
         p.block = 64 * kChainWaves;
         p.task_walkers = (uint64_t)kChainWaves;
         p.grid = (int)((walkers + kChainWaves - 1) / kChainWaves);
-        p.lds = base;
+        p.lds = r->chain_lds; // the table, then each wave's 64 step rows
         return p;
     }
     const int colour_bytes = r->word_bytes;
@@ -699,52 +713,103 @@ static int launch_compiled_kernel(ffr_renderer *r, const LaunchPlan &p, void *fn
 // compile (about half a second) pays off only over repeated renders.
 constexpr uint64_t kCompiledProbeSamples = 1ull << 33;
 
-// Time both kernels on the launch geometry of the coming render, with
-// walkers of at most 1024 samples, into a scratch buffer: the renderer's
-// buffer and stream positions are untouched, and the statistics the probe
-// leaves are reset before the render.  *faster: the compiled kernel took
-// less time (the better of two runs each).
+// Time kernels for the coming render: n launch plans, each with the flame's
+// compiled kernel fns[k] or the table-driven one (nullptr), on the render's
+// walkers with at most 1024 samples each, into a scratch buffer: the
+// renderer's buffer and stream positions are untouched, and the statistics
+// the runs leave are reset before the render.  ms[k]: the better of two
+// runs.
 template <typename R>
-static int probe_compiled(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
-                          uint64_t walker0, uint64_t walkers, uint64_t batch, void *fn,
-                          bool *faster)
+static int time_kernels(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan *plans,
+                        void *const *fns, int n, uint64_t walker0, uint64_t walkers,
+                        uint64_t batch, float *ms)
 {
     const uint64_t pb = std::min<uint64_t>(batch, 1024);
     void *scratch = nullptr;
     HIP_TRY(hipMalloc(&scratch, ffr_buffer_bytes(r)));
-    WalkArgs<R> a;
-    walk_args(r, d, p, walker0, walkers, pb, pb, a);
-    a.buf = scratch;
-    a.bv_limit = ~0ull; // the probe never stops at the bad-value limit
-    std::vector<unsigned char> h(sizeof a);
-    std::memcpy(h.data(), &a, sizeof a);
-    hipError_t e = hipMemcpyAsync(r->dargs, h.data(), sizeof a, hipMemcpyHostToDevice, r->stream);
-    float best[2] = {1e30f, 1e30f};
+    std::vector<std::vector<unsigned char>> h((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        WalkArgs<R> a;
+        walk_args(r, d, plans[k], walker0, walkers, pb, pb, a);
+        a.buf = scratch;
+        a.bv_limit = ~0ull; // the runs never stop at the bad-value limit
+        h[(size_t)k].resize(sizeof a);
+        std::memcpy(h[(size_t)k].data(), &a, sizeof a);
+        ms[k] = 1e30f;
+    }
+    hipError_t e = hipSuccess;
     for (int round = 0; round < 2 && e == hipSuccess; ++round)
-        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-            e = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream) ? hipErrorLaunchFailure
-                                                                        : hipSuccess;
+        for (int k = 0; k < n && e == hipSuccess; ++k) {
+            e = hipMemcpyAsync(r->dargs, h[(size_t)k].data(), h[(size_t)k].size(),
+                               hipMemcpyHostToDevice, r->stream);
+            if (e == hipSuccess)
+                e = launch_reset_stats(r->dstats, r->dxf, r->nxf, r->stream)
+                        ? hipErrorLaunchFailure
+                        : hipSuccess;
             if (e == hipSuccess)
                 e = hipEventRecord(r->ev0, r->stream);
             if (e == hipSuccess)
-                e = (hipError_t)(k ? launch_compiled_kernel(r, p, fn)
-                                   : launch_table_kernel(r, p, (int)d.slot_type.size()));
+                e = (hipError_t)(fns[k] ? launch_compiled_kernel(r, plans[k], fns[k])
+                                        : launch_table_kernel(r, plans[k],
+                                                              (int)d.slot_type.size()));
             if (e == hipSuccess)
                 e = hipEventRecord(r->ev1, r->stream);
             if (e == hipSuccess)
                 e = hipEventSynchronize(r->ev1);
-            float ms = 0;
+            float t = 0;
             if (e == hipSuccess)
-                e = hipEventElapsedTime(&ms, r->ev0, r->ev1);
+                e = hipEventElapsedTime(&t, r->ev0, r->ev1);
             if (e == hipSuccess)
-                best[k] = std::min(best[k], ms);
+                ms[k] = std::min(ms[k], t);
         }
     const hipError_t ef = hipFree(scratch);
     if (e == hipSuccess)
         e = ef;
     if (e != hipSuccess)
-        return fail(FFR_E_DEVICE, std::string("compiled-kernel probe: ") + hipGetErrorString(e));
-    *faster = best[1] < best[0];
+        return fail(FFR_E_DEVICE, std::string("kernel probe: ") + hipGetErrorString(e));
+    return FFR_OK;
+}
+
+// With the cell cache left to the renderer (cell_cache -1), a renderer in
+// steady use (its second render or later, once its renders have asked for
+// this many samples in all) times the cached plan the renderer chose
+// against the same launch without the cache, and keeps the faster per
+// walker count.  (The cache keeps hot cells off the device's atomics but
+// lengthens each walker's chain: the Sierpinski triangle at 64 walkers per
+// CU rendered faster without it.)
+constexpr uint64_t kCacheProbeSamples = 1ull << 28;
+
+template <typename R>
+static int choose_cache(ffr_renderer *r, const DeviceFlame<R> &d, LaunchPlan &p,
+                        uint64_t walker0, uint64_t walkers, uint64_t batch, uint64_t samples)
+{
+    if (r->cache_pref >= 0 || !p.cache || p.chain || p.bin)
+        return FFR_OK;
+    LaunchShape sh = renderer_shape(r);
+    sh.cache = 0;
+    const auto known = r->cache_choice.find(walkers);
+    if (known != r->cache_choice.end()) {
+        if (!known->second)
+            p = plan_launch(r, walkers, batch, sh);
+        return FFR_OK;
+    }
+    // FFR_CACHE_PROBE_SAMPLES lowers the threshold (tests probe small renders)
+    uint64_t threshold = kCacheProbeSamples;
+    if (const char *env = std::getenv("FFR_CACHE_PROBE_SAMPLES"))
+        threshold = std::strtoull(env, nullptr, 10);
+    if (r->launches == 0 || r->samples_asked + samples < threshold)
+        return FFR_OK;
+    const LaunchPlan plans[2] = {p, plan_launch(r, walkers, batch, sh)};
+    void *const fns[2] = {nullptr, nullptr};
+    float ms[2];
+    const int rc = time_kernels(r, d, plans, fns, 2, walker0, walkers, batch, ms);
+    if (rc != FFR_OK)
+        return rc;
+    // the cache stays unless the launch without it is 3 % faster
+    const bool keep = !(ms[1] < 0.97f * ms[0]);
+    r->cache_choice[walkers] = keep;
+    if (!keep)
+        p = plans[1];
     return FFR_OK;
 }
 
@@ -788,10 +853,13 @@ static int prepare_compiled(ffr_renderer *r, const DeviceFlame<R> &d, const Laun
     if (rtc_flame_kernel(a, d.cw, k, r->device, &fn, &compile_s, err) != 0)
         return fail(FFR_E_DEVICE, err);
     if (probe) {
-        bool faster = false;
-        const int rc = probe_compiled(r, d, p, walker0, walkers, batch, fn, &faster);
+        const LaunchPlan plans[2] = {p, p};
+        void *const fns[2] = {nullptr, fn};
+        float ms[2];
+        const int rc = time_kernels(r, d, plans, fns, 2, walker0, walkers, batch, ms);
         if (rc != FFR_OK)
             return rc;
+        const bool faster = ms[1] < ms[0];
         r->compiled_choice[key] = faster;
         if (!faster)
             return FFR_OK;
@@ -952,16 +1020,21 @@ static int launch(ffr_renderer *r, uint64_t samples)
     uint64_t batch, walkers, last;
     split_samples(r, samples, batch, walkers, last);
     LaunchPlan p = plan_launch(r, walkers, batch, renderer_shape(r));
+    // (the cache and compiled-kernel probes run first: the reset after them
+    // clears what they counted)
+    int rc = r->mode == MODE_F64_U64
+                 ? choose_cache(r, r->f64, p, r->next_walker, walkers, batch, samples)
+                 : choose_cache(r, r->f32, p, r->next_walker, walkers, batch, samples);
+    if (rc != FFR_OK)
+        return rc;
     r->last_walkers = walkers;
     r->last_batch = batch;
     r->last_cache = p.cache ? p.slots : 0;
     r->last_chain = p.chain;
     r->last_bin = p.bin;
-    // (the compiled-kernel probe runs first: the reset after it clears what
-    // the probe counted)
-    int rc = r->mode == MODE_F64_U64
-                 ? prepare_compiled(r, r->f64, p, r->next_walker, walkers, batch, last, samples)
-                 : prepare_compiled(r, r->f32, p, r->next_walker, walkers, batch, last, samples);
+    rc = r->mode == MODE_F64_U64
+             ? prepare_compiled(r, r->f64, p, r->next_walker, walkers, batch, last, samples)
+             : prepare_compiled(r, r->f32, p, r->next_walker, walkers, batch, last, samples);
     if (rc != FFR_OK)
         return rc;
     r->last_compiled = r->rtc_fn != nullptr;

@@ -143,6 +143,34 @@ __host__ __device__ constexpr int chain_walkers_per_cu(bool shared, bool post_id
                  : (shared && post_id) ? kChainWalkersPerCu : kChainWalkersPerCuGeneral;
 }
 
+// The wave-per-walker kernel stages each block's 64 steps in LDS, one row
+// of reals per step holding what the serial part reads: the pre-affine
+// offset; the matrix and slot weights unless one shared matrix and weight
+// serve every xform (sh); the post-affine unless every one is the identity
+// (pi); colour speed and colour (nc > 0); the flags where they select
+// (3-D, slots, colour).  Rows are whole 16-byte units.
+__host__ __device__ constexpr int chain_row_reals(int real_bytes, int d, int nc, int ns, bool pi,
+                                                  bool sh)
+{
+    const int n = d + (sh ? 0 : d * d + ns) + (pi ? 0 : d * d + d) + (nc > 0 ? 1 + nc : 0) +
+                  ((d == 3 || !sh || nc > 0) ? 1 : 0);
+    const int per16 = 16 / real_bytes;
+    return (n + per16 - 1) / per16 * per16;
+}
+// rows per wave: the block's 64 steps and two more that the serial part
+// reads ahead past the last step (never used)
+constexpr int kChainRows = 66;
+// the kernel's build for a flame: slots rounded up to 1, 2 or 4; the
+// shared-matrix build takes one slot and identity post-affines
+__host__ __device__ constexpr int chain_build_ns(int slots)
+{
+    return slots <= 1 ? 1 : slots == 2 ? 2 : 4;
+}
+__host__ __device__ constexpr bool chain_build_sh(bool shared, bool post_id, int slots)
+{
+    return shared && post_id && slots <= 1;
+}
+
 #ifndef __HIPCC_RTC__
 // a walker-kernel launch: one flame (args, a device WalkArgs) or a batch
 // (descs, device WalkArgs[] indexed by the tasks' flame numbers)
@@ -193,6 +221,13 @@ __host__ __device__ inline size_t lds_masks_offset(int tab_reals, int real_bytes
 __host__ __device__ inline size_t cache_lds_offset(int tab_reals, int real_bytes, int nxf)
 {
     return lds_masks_offset(tab_reals, real_bytes) + 16u * (size_t)nxf;
+}
+
+// the wave-per-walker kernel's step rows follow the table, masks and
+// counters (16-byte aligned): kChainWaves waves of kChainRows rows
+__host__ __device__ inline size_t chain_stage_offset(int tab_reals, int real_bytes, int nxf)
+{
+    return (cache_lds_offset(tab_reals, real_bytes, nxf) + 15u) & ~(size_t)15u;
 }
 
 // bytes of one cell-cache entry: key, count and the colour words
