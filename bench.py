@@ -82,15 +82,19 @@ C5_3D_SAMPLES = 1 << 30
 C5_CHUNKS = 4
 C5_BUDGET = 200_000_000_000
 HBM_PEAK_GBS = 8000.0
-# walker kernel HBM bytes per launch on the headline workload (FP32):
-# FETCH_SIZE 948.75 KiB x 2.0 + WRITE_SIZE 18634055.5 KiB x 1.0 per dispatch.
-# The factors come from kernels of known traffic on the same buffer (the add
-# reads 2 x 265420800 B and reports 259236 KiB of FETCH_SIZE, the histogram
-# pass reads the buffer once and reports half of it; the clear and the add
-# each write 265420800 B and report exactly 259200 KiB of WRITE_SIZE).
-# Global atomics count 32 B each in WRITE_SIZE.  The raw values are in
-# profiles/walk_kernel_random_color_3840x2160_hbm.md.
-C2_FP32_TRAFFIC_BYTES = 948.75 * 1024 * 2.0 + 18634055.5 * 1024 * 1.0
+# walker kernel HBM bytes per launch on the headline workload, from two
+# counter runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, the timed
+# step's dispatch): FP32 (native trajectories, table-driven kernel) FETCH
+# 901.125 x 2.0 + WRITE 18719063.3125 x 1.0; FP64 (the flame's compiled
+# kernel, which the probe keeps) FETCH 1001.1875 x 2.0 + WRITE
+# 18871357.5625 x 1.0.  The factors come from kernels of known traffic on
+# the same buffer (the add reads 2 x 265420800 B and reports 259236 KiB of
+# FETCH_SIZE, the histogram pass reads the buffer once and reports half of
+# it; the clear and the add each write 265420800 B and report exactly
+# 259200 KiB of WRITE_SIZE).  Global atomics count 32 B each in WRITE_SIZE.
+# The raw values are in profiles/walk_kernel_random_color_3840x2160_hbm.md.
+C2_FP32_TRAFFIC_BYTES = 901.125 * 1024 * 2.0 + 18719063.3125 * 1024 * 1.0
+C2_FP64_TRAFFIC_BYTES = 1001.1875 * 1024 * 2.0 + 18871357.5625 * 1024 * 1.0
 
 
 def log(*a):
@@ -521,6 +525,7 @@ def main():
             fp64, h64 = run_gpu(ffr, c2_text, precision="fp64", samples=C2_SAMPLES, steps=S,
                                 warmup=W, bytes_per_plot=32, seed=C2_SEED, keep=True,
                                 image=True, cpu=not args.no_cpu)
+            fp64["roofline"]["traffic"] = C2_FP64_TRAFFIC_BYTES
             l1_c2 = l1(h32, h64)
             del h32
             # the sampling floor: FP64 against FP64 on other streams

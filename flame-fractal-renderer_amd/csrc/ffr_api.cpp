@@ -69,6 +69,7 @@ struct ffr_renderer {
     bool chain_post_id = false; // ... and every post-affine is the identity
     bool chain_shared = false;  // ... and one pre-affine matrix and weight
     size_t chain_lds = 0;       // its LDS bytes
+    int chain_per_cu = 0;       // FFR_CHAIN_PER_CU: its walkers-per-CU limit (0: the build's)
     size_t lds_bytes = 0;
     // buffer
     void *dbuf = nullptr;
@@ -317,6 +318,8 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
         if (const char *e = std::getenv("FFR_BINNED"))
             r->bin_pref = std::atoi(e);
         r->compiled_pref = opt.compiled < 0 ? -1 : (opt.compiled ? 1 : 0);
+        if (const char *e = std::getenv("FFR_CHAIN_PER_CU"))
+            r->chain_per_cu = std::atoi(e);
         if (const char *e = std::getenv("FFR_COMPILED"))
             r->compiled_pref = std::atoi(e) < 0 ? -1 : (std::atoi(e) ? 1 : 0);
     }
@@ -532,10 +535,12 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
 {
     LaunchPlan p;
     const size_t base = std::max(table_lds(r), base_lds);
+    const int per_cu = r->chain_per_cu > 0 ? r->chain_per_cu
+                                            : chain_walkers_per_cu(r->chain_shared,
+                                                                   r->chain_post_id,
+                                                                   r->spec.color_dims);
     const bool chain = sh.cache < 0 && sh.block <= 0 && base_lds == 0 && r->chain_ok &&
-                       walkers <= (uint64_t)r->cu_count *
-                                      (uint64_t)chain_walkers_per_cu(
-                                          r->chain_shared, r->chain_post_id, r->spec.color_dims);
+                       walkers <= (uint64_t)r->cu_count * (uint64_t)per_cu;
     // binned accumulation: single renders of count-only buffers under 2^28
     // cells whose walkers all fit one pass's records; chosen from 2^22 cells
     // on when the lane-per-walker kernel runs (the samples spread beyond
@@ -718,11 +723,12 @@ constexpr uint64_t kCompiledProbeSamples = 1ull << 33;
 // walkers with at most 1024 samples each, into a scratch buffer: the
 // renderer's buffer and stream positions are untouched, and the statistics
 // the runs leave are reset before the render.  ms[k]: the better of two
-// runs.
+// runs (one, when after the first every other kernel took more than
+// stop_ratio times kernel 0's time; 0: always two).
 template <typename R>
 static int time_kernels(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan *plans,
                         void *const *fns, int n, uint64_t walker0, uint64_t walkers,
-                        uint64_t batch, float *ms)
+                        uint64_t batch, float *ms, float stop_ratio = 0.f)
 {
     const uint64_t pb = std::min<uint64_t>(batch, 1024);
     void *scratch = nullptr;
@@ -738,7 +744,14 @@ static int time_kernels(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPl
         ms[k] = 1e30f;
     }
     hipError_t e = hipSuccess;
-    for (int round = 0; round < 2 && e == hipSuccess; ++round)
+    for (int round = 0; round < 2 && e == hipSuccess; ++round) {
+        if (round == 1 && stop_ratio > 0.f) {
+            bool all_slower = true;
+            for (int k = 1; k < n; ++k)
+                all_slower = all_slower && ms[k] > stop_ratio * ms[0];
+            if (all_slower)
+                break;
+        }
         for (int k = 0; k < n && e == hipSuccess; ++k) {
             e = hipMemcpyAsync(r->dargs, h[(size_t)k].data(), h[(size_t)k].size(),
                                hipMemcpyHostToDevice, r->stream);
@@ -762,6 +775,7 @@ static int time_kernels(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPl
             if (e == hipSuccess)
                 ms[k] = std::min(ms[k], t);
         }
+    }
     const hipError_t ef = hipFree(scratch);
     if (e == hipSuccess)
         e = ef;
@@ -772,12 +786,16 @@ static int time_kernels(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPl
 
 // With the cell cache left to the renderer (cell_cache -1), a renderer in
 // steady use (its second render or later, once its renders have asked for
-// this many samples in all) times the cached plan the renderer chose
-// against the same launch without the cache, and keeps the faster per
-// walker count.  (The cache keeps hot cells off the device's atomics but
-// lengthens each walker's chain: the Sierpinski triangle at 64 walkers per
-// CU rendered faster without it.)
+// this many samples in all) with at most kCacheProbeWalkersPerCu walkers
+// per CU times the cached plan the renderer chose against the same launch
+// without the cache, and keeps the faster per walker count.  (The cache
+// keeps hot cells off the device's atomics but lengthens each walker's
+// chain, which bounds renders of few walkers: the Sierpinski triangle at
+// 64 walkers per CU rendered faster without it.  With many walkers the
+// atomics bound instead: the headline flame's 2048 walkers per CU ran 40
+// times slower without the cache, so there is nothing to probe.)
 constexpr uint64_t kCacheProbeSamples = 1ull << 28;
+constexpr uint64_t kCacheProbeWalkersPerCu = 256;
 
 template <typename R>
 static int choose_cache(ffr_renderer *r, const DeviceFlame<R> &d, LaunchPlan &p,
@@ -797,12 +815,14 @@ static int choose_cache(ffr_renderer *r, const DeviceFlame<R> &d, LaunchPlan &p,
     uint64_t threshold = kCacheProbeSamples;
     if (const char *env = std::getenv("FFR_CACHE_PROBE_SAMPLES"))
         threshold = std::strtoull(env, nullptr, 10);
-    if (r->launches == 0 || r->samples_asked + samples < threshold)
+    if (r->launches == 0 || r->samples_asked + samples < threshold ||
+        walkers > kCacheProbeWalkersPerCu * (uint64_t)r->cu_count)
         return FFR_OK;
     const LaunchPlan plans[2] = {p, plan_launch(r, walkers, batch, sh)};
     void *const fns[2] = {nullptr, nullptr};
     float ms[2];
-    const int rc = time_kernels(r, d, plans, fns, 2, walker0, walkers, batch, ms);
+    // (one run each when the launch without the cache is far slower)
+    const int rc = time_kernels(r, d, plans, fns, 2, walker0, walkers, batch, ms, 1.5f);
     if (rc != FFR_OK)
         return rc;
     // the cache stays unless the launch without it is 3 % faster

@@ -35,7 +35,7 @@ int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s);
 // tiles (cells < 2^28).
 constexpr int kBinTileBits = 15;
 constexpr int kBinMaxTiles = 8192;
-constexpr unsigned long long kBinRecordsPerBlock = 1ull << 16;
+constexpr unsigned long long kBinRecordsPerBlock = 1ull << 18;
 // records of one pass (1 GiB; the sorted copy takes as much again)
 constexpr unsigned long long kBinPassRecords = 1ull << 28;
 struct BinScratch {

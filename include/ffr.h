@@ -56,11 +56,11 @@ typedef struct ffr_options {
     uint64_t bad_value_cap;  /* bad-value records kept per render (default 4096)      */
     /* launch shape (no counterpart in the reference; results do not depend
      * on it): accumulation through the per-workgroup LDS cell cache
-     * (-1 chosen per render, and once a renderer renders again after its
-     * renders have asked for 2^28 samples in all, by a timed probe of the
-     * launch with and without it per walker count; 0 off; 1 on where the
-     * flame allows it), the LDS the cache may take in KiB (0: 144), lanes
-     * per workgroup (0: chosen) */
+     * (-1 chosen per render, and for renders of at most 256 walkers per CU,
+     * once a renderer renders again after its renders have asked for 2^28
+     * samples in all, by a timed probe of the launch with and without it
+     * per walker count; 0 off; 1 on where the flame allows it), the LDS the
+     * cache may take in KiB (0: 144), lanes per workgroup (0: chosen) */
     int32_t cell_cache;
     int32_t cache_kib;
     int32_t block;
