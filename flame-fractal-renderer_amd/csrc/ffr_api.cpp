@@ -535,10 +535,9 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
 {
     LaunchPlan p;
     const size_t base = std::max(table_lds(r), base_lds);
-    const int per_cu = r->chain_per_cu > 0 ? r->chain_per_cu
-                                            : chain_walkers_per_cu(r->chain_shared,
-                                                                   r->chain_post_id,
-                                                                   r->spec.color_dims);
+    const int per_cu = r->chain_per_cu > 0
+                           ? r->chain_per_cu
+                           : chain_walkers_per_cu(r->chain_post_id, r->spec.color_dims);
     const bool chain = sh.cache < 0 && sh.block <= 0 && base_lds == 0 && r->chain_ok &&
                        walkers <= (uint64_t)r->cu_count * (uint64_t)per_cu;
     // binned accumulation: single renders of count-only buffers under 2^28

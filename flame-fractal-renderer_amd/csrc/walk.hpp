@@ -124,23 +124,21 @@ constexpr int kHotSlots = 1 << 15;
 // the wave-per-walker kernel (chain_kernel.inc): variation slots per xform
 // it holds, walkers (waves) per workgroup, and the most walkers per CU it
 // takes (beyond, the lane-per-walker kernel has enough waves in flight).
-// The limit depends on what its serial step broadcasts.  Measured at 8, 16
-// and 32 walkers of 16384 per CU against the lane kernel: the build that
-// holds one shared matrix and weight with identity post-affines wins at 32
-// (Sierpinski FP64 10.7 ms against 15.8); builds that broadcast every
-// xform's matrix or read post-affines from the table lose at 32 (three
-// linear slots with post-affines: 29.4 against 22.6 ms) and win at 16
-// (15.4 against 22.6); colour builds lose at 16 with post-affines (43.1
-// against 39.3 ms) and win at 8 (22.0 against 26.5).
+// The limit depends on the length of its serial step.  Measured at 8 to 128
+// walkers of 16384 per CU against the lane kernel (kernel ms, FP64): with
+// identity post-affines and no colour it wins up to 64 (Sierpinski 19.9
+// against 25.3, uneven weights 30.3 against 32.9) and breaks even at 128
+// (39.3 against 40.7, 59.9 against 59.6); with post-affines from the table
+// it wins at 16 (three linear slots 12.0 against 22.7) and loses at 32
+// (23.6 against 22.8); colour builds win at 16 (61.2 against 63.8, with
+// post-affines 38.4 against 39.1) and break even at 32.
 constexpr int kChainSlots = 4;
 constexpr int kChainWaves = 4;
-constexpr int kChainWalkersPerCu = 32;      // shared matrix, identity post, no colour
-constexpr int kChainWalkersPerCuGeneral = 16; // other builds without colour
-constexpr int kChainWalkersPerCuColour = 8;   // colour builds
-__host__ __device__ constexpr int chain_walkers_per_cu(bool shared, bool post_id, int r)
+constexpr int kChainWalkersPerCuPostId = 64; // identity post-affines, no colour
+constexpr int kChainWalkersPerCuGeneral = 16; // post-affines from the table, or colour
+__host__ __device__ constexpr int chain_walkers_per_cu(bool post_id, int r)
 {
-    return r > 0 ? kChainWalkersPerCuColour
-                 : (shared && post_id) ? kChainWalkersPerCu : kChainWalkersPerCuGeneral;
+    return (r == 0 && post_id) ? kChainWalkersPerCuPostId : kChainWalkersPerCuGeneral;
 }
 
 // The wave-per-walker kernel stages each block's 64 steps in LDS, one row

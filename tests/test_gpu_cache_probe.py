@@ -44,9 +44,10 @@ def same(a, b, dims):
 
 CASES = {
     # name: (flame, dims, precision, walkers, samples per walker)
-    # 64 walkers per CU of a linear flame on a 1024^2 buffer: the cache
-    # lengthens each walker's chain
-    "sierpinski_1024": (lambda: read_flame("sierpinski_1024.json"), 2, "fp64", 16384, 512),
+    # 128 walkers per CU of a linear flame on a 1024^2 buffer (beyond the
+    # wave-per-walker kernel's limit): the cache lengthens each walker's
+    # chain
+    "sierpinski_1024": (lambda: read_flame("sierpinski_1024.json"), 2, "fp64", 32768, 256),
     # a concentrated colour flame: the cache takes most cell updates
     "headline_384": (lambda: read_flame("random_color_3840x2160.json").replace(
         "3840,\n  2160", "384,\n  216"), 2, "fp32", 32768, 256),
@@ -82,8 +83,8 @@ def test_cache_probe_not_when_forced(ffr_mod, low_probe_threshold):
     """a cache forced on or off is never probed"""
     text = read_flame("sierpinski_1024.json")
     for cache, want in ((1, True), (0, False)):
-        r = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=512, cell_cache=cache, compiled=0)
+        r = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=256, cell_cache=cache, compiled=0)
         for _ in range(3):
-            assert r.render(16384 * 512)
+            assert r.render(32768 * 256)
             assert (r.last_stats.cache_slots > 0) == want
         r.close()

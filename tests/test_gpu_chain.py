@@ -329,12 +329,12 @@ def test_chain_bad_value_limit_stops(ffr_mod):
 
 
 def test_chain_at_walker_limit_and_cumulative(ffr_mod):
-    """the most walkers the kernel takes (32 per CU), then a second render
+    """the most walkers the kernel takes (64 per CU), then a second render
     onto the same buffer: cumulative statistics equal the lane kernel's"""
     text = sierpinski(256)
     a = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=300)
     b = ffr_mod.BufferRenderer(text, seed=SEED, batch_size=300, cell_cache=0)
-    walkers = 32 * ffr_mod.DeviceMemory.cu_count()
+    walkers = 64 * ffr_mod.DeviceMemory.cu_count()
     for rep in range(2):
         print(f"walker limit {walkers}, render {rep}", flush=True)
         assert a.render(walkers * 300) and b.render(walkers * 300)
@@ -347,12 +347,12 @@ def test_chain_at_walker_limit_and_cumulative(ffr_mod):
     b.close()
 
 
-@pytest.mark.parametrize("name,per_cu", [("sierpinski", 32), ("uneven_weights", 16),
-                                         ("three_slots", 16), ("colour", 8)])
+@pytest.mark.parametrize("name,per_cu", [("sierpinski", 64), ("uneven_weights", 64),
+                                         ("three_slots", 16), ("colour", 16)])
 def test_chain_cutoff_per_build(ffr_mod, name, per_cu):
     """the most walkers per CU the wave-per-walker kernel takes depends on
-    what its serial step broadcasts: 32 for one shared matrix with identity
-    post-affines, 16 for the other builds, 8 with colour"""
+    the length of its serial step: 64 with identity post-affines and no
+    colour, 16 with post-affines from the table or with colour"""
     text = TF.color_flame(3) if name == "colour" else FLAMES[name][0]()
     cu = ffr_mod.DeviceMemory.cu_count()
     for walkers, want in ((per_cu * cu, 1), (per_cu * cu + 1, 0)):
