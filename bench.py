@@ -571,10 +571,10 @@ def main():
                          warmup=W, bytes_per_plot=4, seed=C3_SEED)
         if res["binned"]:
             # binned accumulation moves records, not atomics: per sample 4 B
-            # written by the walker, the sentinel fill, the tile histogram's
-            # read, the scatter's read and write, the accumulation's read
-            res["record_bytes_per_sample"] = 24
-            res["record_traffic_GBs"] = (24 * C3_SAMPLES / (res["kernel_ms"] * 1e-3) / 1e9)
+            # written by the walker, the tile histogram's read, the scatter's
+            # read and write, the accumulation's read
+            res["record_bytes_per_sample"] = 20
+            res["record_traffic_GBs"] = (20 * C3_SAMPLES / (res["kernel_ms"] * 1e-3) / 1e9)
         workloads["tetrahedron_512_fp32"] = res
         log("tetrahedron_512_fp32", json.dumps(res))
 

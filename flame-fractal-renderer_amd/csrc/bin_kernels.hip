@@ -10,7 +10,7 @@
 // whatever their scope; a buffer far beyond the LDS whose samples spread
 // over more cells than an LDS cache holds is bound by that rate.  Records
 // move at HBM bandwidth instead:
-//   walker   writes 4 B per sample (records pre-filled with kEmptyKey)
+//   walker   writes 4 B per sample (kEmptyKey where nothing is plotted)
 //   hist     reads the records, one tile histogram per block of records
 //   colscan  per tile, the running sum of the blocks' counts (the block's
 //            first slot in the tile's region) and the tile's total
@@ -18,7 +18,9 @@
 //   scatter  re-reads each block in chunks, sorts a chunk by tile in LDS and
 //            writes each tile's run of it to the tile's region (consecutive
 //            lanes, consecutive addresses)
-//   accum    one workgroup per tile: counts in LDS, then one add per cell
+//   accum    per tile, one workgroup per chunk of up to kAccumChunk of the
+//            tile's records: counts in LDS, then one add per cell (an
+//            atomic one where chunks share the tile)
 // A tile is kBinTileCells consecutive cells (dimension 0 fastest), whose
 // u32 counts fit the LDS of one workgroup.
 #include "util_kernels.hpp"
@@ -33,14 +35,6 @@ namespace {
 
 constexpr int kBinBlock = 256;                       // lanes per workgroup
 constexpr unsigned kBinEmpty = kEmptyKey;
-
-__global__ void __launch_bounds__(kBinBlock)
-    bin_fill_kernel(unsigned *rec, unsigned long long n)
-{
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (unsigned long long)gridDim.x * blockDim.x)
-        rec[i] = kBinEmpty;
-}
 
 // records each lane loads before it uses them (independent loads in flight)
 constexpr int kBinUnroll = 8;
@@ -235,23 +229,68 @@ __global__ void __launch_bounds__(kScatterLanes)
     }
 }
 
-// one workgroup per tile: count the tile's records in LDS, then add each
-// cell's count into the buffer (no other workgroup touches the tile)
+// The LDS word of cell c of a tile: c with its low five bits (its bank)
+// turned by a hash of the rest.  Cells that flames occupy often share their
+// low bits (the Sierpinski tetrahedron's occupied cells of a row have x
+// digits that avoid y's and z's, so a few banks held them all and the
+// counts' atomics queued on them); the hash spreads each group of 32 cells
+// over the banks, one bank each.
+__device__ __forceinline__ unsigned tile_word(unsigned c)
+{
+    return c ^ (((c >> 5) * 0x9E3779B1u) >> 27);
+}
+
+// Tiles take very different shares of the records (the tetrahedron's
+// lowest slices hold thousands of occupied cells, its highest a few), and
+// one workgroup per tile left the device waiting on the largest.  So a tile
+// is accumulated in chunks of at most kAccumChunk records, a workgroup
+// each: chunks[t] per tile, then their exclusive scan gives tile t's work
+// items cstart[t] .. cstart[t + 1].
+constexpr unsigned kAccumChunk = 1u << 17;
+
+__global__ void __launch_bounds__(kBinBlock)
+    bin_chunks_kernel(const unsigned *start, int ntiles, unsigned *chunks)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles)
+        chunks[t] = (start[t + 1] - start[t] + kAccumChunk - 1) / kAccumChunk;
+}
+
+// work item w: chunk w - cstart[t] of the tile t whose items hold w.  The
+// chunk's counts go to LDS, then each cell's count into the buffer: a plain
+// add when the tile has one chunk (no other workgroup touches it), an
+// atomic one when several chunks share the tile.
 template <typename CT>
 __global__ void __launch_bounds__(1024)
-    bin_accum_kernel(const unsigned *sorted, const unsigned *start, int tile_bits,
-                     unsigned long long cells, CT *buf)
+    bin_accum_kernel(const unsigned *sorted, const unsigned *start, const unsigned *cstart,
+                     int ntiles, int tile_bits, unsigned long long cells, CT *buf)
 {
     extern __shared__ unsigned cnt[];
-    const int t = blockIdx.x;
-    const unsigned s0 = start[t], s1 = start[t + 1];
-    if (s0 == s1)
-        return; // no record in this tile (uniform across the workgroup)
+    const unsigned w = blockIdx.x;
+    if (w >= cstart[ntiles])
+        return; // past the last work item (uniform across the workgroup)
+    // the tile: the last t with cstart[t] <= w (tiles without records have
+    // no items, so their cstart equals the next tile's)
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cstart[mid] <= w)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int t = lo;
+    const unsigned k = w - cstart[t];
+    const bool shared = cstart[t + 1] - cstart[t] > 1;
+    const unsigned s0 = start[t] + k * kAccumChunk;
+    const unsigned s1 = min(start[t + 1], s0 + kAccumChunk);
     const unsigned tile = 1u << tile_bits;
     const unsigned long long c0 = (unsigned long long)t << tile_bits;
     const unsigned long long left = cells - c0;
     const unsigned ncell = left < tile ? (unsigned)left : tile;
-    for (unsigned i = threadIdx.x; i < ncell; i += blockDim.x)
+    // (whole groups of 32: tile_word keeps a cell in its group)
+    const unsigned nzero = (ncell + 31u) & ~31u;
+    for (unsigned i = threadIdx.x; i < nzero; i += blockDim.x)
         cnt[i] = 0;
     __syncthreads();
     const unsigned mask = tile - 1u;
@@ -266,12 +305,16 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
         for (int j = 0; j < kBinUnroll; ++j)
             if (c[j] != kBinEmpty)
-                atomicAdd(&cnt[c[j] & mask], 1u);
+                atomicAdd(&cnt[tile_word(c[j] & mask)], 1u);
     }
     __syncthreads();
     for (unsigned i = threadIdx.x; i < ncell; i += blockDim.x) {
-        const unsigned v = cnt[i];
-        if (v)
+        const unsigned v = cnt[tile_word(i)];
+        if (!v)
+            continue;
+        if (shared)
+            atomicAdd(&buf[c0 + i], (CT)v);
+        else
             buf[c0 + i] += (CT)v;
     }
 }
@@ -288,13 +331,6 @@ size_t bin_scratch_hist_words(unsigned long long records, int ntiles)
 {
     const unsigned long long blocks = (records + kBinRecordsPerBlock - 1) / kBinRecordsPerBlock;
     return (size_t)std::max<unsigned long long>(blocks, 1) * (size_t)ntiles;
-}
-
-int launch_bin_fill(unsigned *rec, unsigned long long n, hipStream_t s)
-{
-    hipLaunchKernelGGL(bin_fill_kernel, dim3(grid_for(n, kBinBlock)), dim3(kBinBlock), 0, s, rec,
-                       n);
-    return (int)hipGetLastError();
 }
 
 int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,
@@ -319,14 +355,22 @@ int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,
         hipLaunchKernelGGL(bin_scatter_kernel<16>, dim3(nblocks), dim3(kScatterLanes),
                            scatter_lds(ntiles, 16), s, b.rec, n_records, per, kBinTileBits, ntiles,
                            b.hist, b.start, b.sorted);
+    // accumulation work items: per tile, chunks of at most kAccumChunk
+    // records (at most ntiles + n_records / kAccumChunk items in all)
+    unsigned *chunks = b.total, *cstart = b.total + ntiles + 1;
+    hipLaunchKernelGGL(bin_chunks_kernel, dim3((ntiles + kBinBlock - 1) / kBinBlock),
+                       dim3(kBinBlock), 0, s, b.start, ntiles, chunks);
+    hipLaunchKernelGGL(bin_tilescan_kernel, dim3(1), dim3(1024), 0, s, chunks, ntiles, cstart);
+    const unsigned long long items = (unsigned long long)ntiles + n_records / kAccumChunk;
     const size_t tile_lds = sizeof(unsigned) << kBinTileBits;
     if (word_bytes == 8)
-        hipLaunchKernelGGL((bin_accum_kernel<unsigned long long>), dim3(ntiles), dim3(1024),
-                           tile_lds, s, b.sorted, b.start, kBinTileBits, cells,
-                           (unsigned long long *)buf);
+        hipLaunchKernelGGL((bin_accum_kernel<unsigned long long>), dim3((unsigned)items),
+                           dim3(1024), tile_lds, s, b.sorted, b.start, cstart, ntiles,
+                           kBinTileBits, cells, (unsigned long long *)buf);
     else
-        hipLaunchKernelGGL((bin_accum_kernel<unsigned>), dim3(ntiles), dim3(1024), tile_lds, s,
-                           b.sorted, b.start, kBinTileBits, cells, (unsigned *)buf);
+        hipLaunchKernelGGL((bin_accum_kernel<unsigned>), dim3((unsigned)items), dim3(1024),
+                           tile_lds, s, b.sorted, b.start, cstart, ntiles, kBinTileBits, cells,
+                           (unsigned *)buf);
     return (int)hipGetLastError();
 }
 

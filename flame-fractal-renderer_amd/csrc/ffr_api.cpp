@@ -68,6 +68,7 @@ struct ffr_renderer {
     bool chain_ok = false;
     bool chain_post_id = false; // ... and every post-affine is the identity
     bool chain_shared = false;  // ... and one pre-affine matrix and weight
+    bool chain_diag = false;    // ... and that matrix is diagonal
     size_t chain_lds = 0;       // its LDS bytes
     int chain_per_cu = 0;       // FFR_CHAIN_PER_CU: its walkers-per-CU limit (0: the build's)
     size_t lds_bytes = 0;
@@ -221,6 +222,12 @@ static int setup_tables(ffr_renderer *r, const DeviceFlame<R> &d)
         if (std::memcmp(w, d.table.data() + d.slot_off[0], sizeof(R)))
             r->chain_shared = false;
     }
+    // the shared matrix is diagonal (off-diagonal entries +0 or -0): the
+    // kernel leaves out their products, which are zeros for a finite point
+    r->chain_diag = r->chain_shared;
+    for (int i = 0; i < d.dims * d.dims && r->chain_diag; ++i)
+        if (i / d.dims != i % d.dims && d.table[(size_t)(d.xf0 + d.lay.preA + i)] != (R)0)
+            r->chain_diag = false;
     // its LDS: the table, then kChainRows step rows per wave of the flame's
     // build
     const int slots = (int)d.slot_type.size();
@@ -692,6 +699,7 @@ static int launch_table_kernel(ffr_renderer *r, const LaunchPlan &p, int nslots)
     L.chain_slots = nslots;
     L.chain_post_id = r->chain_post_id;
     L.chain_shared = r->chain_shared;
+    L.chain_diag = r->chain_diag;
     L.cache = p.cache;
     L.grid = p.grid;
     L.block = p.block;
@@ -904,20 +912,7 @@ static int launch_single(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
         return (int)e;
     if (r->rtc_fn)
         return launch_compiled_kernel(r, p, r->rtc_fn);
-    WalkLaunch L;
-    L.nc = r->spec.color_dims;
-    L.vs = r->vs;
-    L.chain = p.chain;
-    L.chain_slots = (int)d.slot_type.size();
-    L.chain_post_id = r->chain_post_id;
-    L.chain_shared = r->chain_shared;
-    L.cache = p.cache;
-    L.grid = p.grid;
-    L.block = p.block;
-    L.lds = p.lds;
-    L.stream = r->stream;
-    L.args = r->dargs;
-    return launch_walk_any(r->mode, r->spec.dims, L);
+    return launch_table_kernel(r, p, (int)d.slot_type.size());
 }
 
 // walkers and samples per walker of a render of `samples`
@@ -977,14 +972,16 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
         e = ensure_device((void **)&r->bin.hist, hist_bytes,
                           bin_scratch_hist_words(cap, (int)ntiles) * 4);
     r->bin_hist_words = hist_bytes / 4;
+    // per tile: records and (after the scatter) accumulation chunks, then
+    // the chunks' first work items (2 (ntiles + 1) words); the regions
     size_t tile_bytes = r->bin_tiles * 4;
-    if (e == hipSuccess && tile_bytes < (ntiles + 1) * 4) {
+    if (e == hipSuccess && tile_bytes < 2 * (ntiles + 1) * 4) {
         size_t t2 = tile_bytes;
-        e = ensure_device((void **)&r->bin.total, tile_bytes, (ntiles + 1) * 4);
+        e = ensure_device((void **)&r->bin.total, tile_bytes, 2 * (ntiles + 1) * 4);
         if (e == hipSuccess)
-            e = ensure_device((void **)&r->bin.start, t2, (ntiles + 1) * 4);
+            e = ensure_device((void **)&r->bin.start, t2, 2 * (ntiles + 1) * 4);
     }
-    r->bin_tiles = e == hipSuccess ? std::max(r->bin_tiles, ntiles + 1) : 0;
+    r->bin_tiles = e == hipSuccess ? std::max(r->bin_tiles, 2 * (ntiles + 1)) : 0;
     if (e == hipSuccess && npass > 1)
         e = ensure_device(&r->wstate, r->wstate_bytes, (size_t)slots * sizeof(WalkerState<R>));
     size_t args_bytes = r->dargs_bin_n * sizeof(WalkArgs<R>);
@@ -1010,10 +1007,8 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     if (e != hipSuccess)
         return (int)e;
     for (uint64_t k = 0; k < npass; ++k) {
+        // (the walker kernel writes every record of the pass)
         const uint64_t n_rec = slots * (std::min<uint64_t>(batch, (k + 1) * span) - k * span);
-        int rc = launch_bin_fill(r->bin.rec, n_rec, r->stream);
-        if (rc)
-            return rc;
         WalkLaunch L;
         L.nc = 0;
         L.vs = r->vs;
@@ -1023,7 +1018,7 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
         L.lds = p.lds;
         L.stream = r->stream;
         L.args = (const unsigned char *)r->dargs_bin + k * sizeof(WalkArgs<R>);
-        rc = launch_walk_any(r->mode, r->spec.dims, L);
+        int rc = launch_walk_any(r->mode, r->spec.dims, L);
         if (rc)
             return rc;
         rc = launch_bin_accumulate(r->bin, n_rec, r->cells, r->dbuf, (int)r->word_bytes,

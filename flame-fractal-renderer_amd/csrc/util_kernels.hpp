@@ -29,9 +29,9 @@ int launch_scatter_atomics(void *buf, unsigned long long cells, int cw, int word
 int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s);
 
 // Binned accumulation of count-only buffers (bin_kernels.hip): the walker
-// writes each plotted sample's cell index into `rec` (pre-filled with
-// kEmptyKey); the records are sorted by tile and each tile's counts are
-// added from LDS.  A tile is 2^kBinTileBits cells; at most kBinMaxTiles
+// writes each plotted sample's cell index into `rec` (kEmptyKey for the
+// others); the records are sorted by tile and each tile's counts are added
+// from LDS.  A tile is 2^kBinTileBits cells; at most kBinMaxTiles
 // tiles (cells < 2^28).
 constexpr int kBinTileBits = 15;
 constexpr int kBinMaxTiles = 8192;
@@ -42,12 +42,12 @@ struct BinScratch {
     unsigned *rec = nullptr;     // records of one pass (cell index or kEmptyKey)
     unsigned *sorted = nullptr;  // the same records grouped by tile
     unsigned *hist = nullptr;    // per block of records, per tile: count, then offset
-    unsigned *total = nullptr;   // per tile: records
+    unsigned *total = nullptr;   // per tile: records; then accumulation chunks and
+                                 // their first work items (2 (ntiles + 1) words)
     unsigned *start = nullptr;   // per tile: first slot of its region (ntiles + 1)
     unsigned long long capacity = 0; // records rec and sorted hold
 };
 size_t bin_scratch_hist_words(unsigned long long records, int ntiles);
-int launch_bin_fill(unsigned *rec, unsigned long long n, hipStream_t s);
 // sort the n_records records of b.rec by tile and add the counts into the
 // buffer (cells count words of word_bytes, no colour words)
 int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,

@@ -188,6 +188,7 @@ struct WalkLaunch {
     int chain_slots = 0;              // its variation slots (1..kChainSlots)
     bool chain_post_id = false;       // every xform's post-affine is the identity
     bool chain_shared = false;        // one pre-affine matrix and weight for all
+    bool chain_diag = false;          // ... and that matrix diagonal
     bool bin = false;                 // binned accumulation (walk_bin_kernel)
 };
 #endif // __HIPCC_RTC__

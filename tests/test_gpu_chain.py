@@ -6,12 +6,14 @@ negzero_post) read the post-affine from the table; maps that share one
 matrix and weight (sierpinski, tetrahedron) take the build that holds them
 once, the others (uneven_weights, fern, ...) broadcast every coefficient.
 
-A render with the cell cache left to the renderer (-1) and at most 32
-walkers per CU takes the wave-per-walker kernel; `cell_cache=0` forces the
-lane-per-walker kernel on the same walkers and streams.  Buffers, samples
-iterated and plotted, xform counts, point extremes and bad values must be
-equal bit for bit, in every precision mode and at walker shapes that leave
-partial blocks of 64 iterations (short walkers, a short last walker).
+A render with the cell cache left to the renderer (-1) and at most the
+build's walkers per CU takes the wave-per-walker kernel; `cell_cache=0`
+forces the lane-per-walker kernel on the same walkers and streams.
+Buffers, samples iterated and plotted, xform counts, point extremes
+(compared as numbers: the kernel leaves out operations that change only
+the sign of a zero) and bad values must be equal, in every precision mode
+and at walker shapes that leave partial blocks of 64 iterations (short
+walkers, a short last walker).
 """
 
 import json
@@ -89,8 +91,20 @@ def uneven_weights(size=96):
     return json.dumps(d)
 
 
+def shared_matrix(A, size=96):
+    """Sierpinski's offsets under one shared pre-affine matrix A: the kernel
+    holds the matrix once; a diagonal one (off-diagonal entries +0 or -0)
+    takes the build that leaves the off-diagonal products out"""
+    d = load(sierpinski(size))
+    for x in d["xforms"]:
+        x["pre_affine"]["A"] = A
+    return json.dumps(d)
+
+
 FLAMES = {
     "sierpinski": (sierpinski, 2),
+    "shared_rotation": (lambda: shared_matrix([[0.4, -0.3], [0.3, 0.4]]), 2),
+    "shared_negzero_diagonal": (lambda: shared_matrix([[0.5, -0.0], [-0.0, 0.45]]), 2),
     "uneven_weights": (uneven_weights, 2),
     "negzero_post": (negzero_post, 2),
     "fern": (TF.fern, 2),
@@ -151,7 +165,8 @@ def test_chain_equals_lane_kernel(ffr_mod, name, mode):
 
 
 @pytest.mark.parametrize("flame", ["three_slots", "negzero_post", "sierpinski",
-                                   "uneven_weights", "rotations_final"])
+                                   "uneven_weights", "rotations_final", "shared_rotation",
+                                   "shared_negzero_diagonal"])
 @pytest.mark.parametrize("mode", ["fp64", "fp32"])
 def test_chain_matches_oracle(ffr_mod, oracle_mod, mode, flame):
     text = FLAMES[flame][0]()
