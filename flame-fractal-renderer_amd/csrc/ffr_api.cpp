@@ -96,6 +96,8 @@ struct ffr_renderer {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the binned walker's probe of the compiled kernel: a pass each
+    hipEvent_t ev_ab[4] = {nullptr, nullptr, nullptr, nullptr};
     uint64_t next_walker = 0;
     // cumulative statistics (getters of the reference)
     uint64_t s_iter = 0, s_plot = 0;
@@ -419,6 +421,9 @@ void ffr_destroy(ffr_renderer *r)
         (void)hipEventDestroy(r->ev0);
     if (r->ev1)
         (void)hipEventDestroy(r->ev1);
+    for (hipEvent_t ev : r->ev_ab)
+        if (ev)
+            (void)hipEventDestroy(ev);
     if (r->own_stream)
         (void)hipStreamDestroy(r->own_stream);
     delete r;
@@ -944,7 +949,8 @@ static hipError_t ensure_device(void **p, size_t &have, size_t bytes)
 // added into the buffer before the next pass overwrites them.
 template <typename R>
 static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchPlan &p,
-                         uint64_t walker0, uint64_t walkers, uint64_t batch, uint64_t last)
+                         uint64_t walker0, uint64_t walkers, uint64_t batch, uint64_t last,
+                         uint64_t samples)
 {
     const uint64_t tw = p.task_walkers;
     const uint64_t slots = (walkers + tw - 1) / tw * tw; // walker slots of the tasks
@@ -1006,26 +1012,90 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
                        r->stream);
     if (e != hipSuccess)
         return (int)e;
+    // The flame's compiled walker (compiled 1; with -1 as for the lane
+    // kernel: a renderer in steady use compiles it, and a render of three
+    // passes or more times its first pass with the table-driven walker and
+    // its second with the compiled one, then keeps the faster for the rest
+    // and for later renders).  Both give the same records.
+    void *fn = nullptr;
+    bool ab = false;
+    const int key = 0x100 | (p.block << 9); // (apart from the lane kernel's keys)
+    if (r->compiled_pref != 0) {
+        const auto known = r->compiled_choice.find(key);
+        uint64_t threshold = kCompiledProbeSamples;
+        if (const char *env = std::getenv("FFR_COMPILED_PROBE_SAMPLES"))
+            threshold = std::strtoull(env, nullptr, 10);
+        const bool want =
+            r->compiled_pref > 0 || (known != r->compiled_choice.end() && known->second);
+        ab = r->compiled_pref < 0 && known == r->compiled_choice.end() && npass >= 3 &&
+             r->launches > 0 && r->samples_asked + samples >= threshold;
+        if (want || ab) {
+            WalkArgs<R> a;
+            walk_args(r, d, p, walker0, walkers, batch, last, a);
+            RtcKernel k;
+            k.mode = r->mode;
+            k.dims = r->spec.dims;
+            k.vs = r->vs;
+            k.block_max = kBinWalkBlock;
+            k.bin = true;
+            std::string err;
+            double compile_s = 0;
+            if (rtc_flame_kernel(a, d.cw, k, r->device, &fn, &compile_s, err) != 0)
+                return fail(FFR_E_DEVICE, err);
+        }
+        if (ab)
+            for (hipEvent_t &ev : r->ev_ab)
+                if (!ev)
+                    HIP_TRY(hipEventCreate(&ev));
+    }
+    void *use = ab ? nullptr : fn;
     for (uint64_t k = 0; k < npass; ++k) {
         // (the walker kernel writes every record of the pass)
         const uint64_t n_rec = slots * (std::min<uint64_t>(batch, (k + 1) * span) - k * span);
-        WalkLaunch L;
-        L.nc = 0;
-        L.vs = r->vs;
-        L.bin = true;
-        L.grid = p.grid;
-        L.block = p.block;
-        L.lds = p.lds;
-        L.stream = r->stream;
-        L.args = (const unsigned char *)r->dargs_bin + k * sizeof(WalkArgs<R>);
-        int rc = launch_walk_any(r->mode, r->spec.dims, L);
+        const unsigned char *args = (const unsigned char *)r->dargs_bin + k * sizeof(WalkArgs<R>);
+        if (ab && k == 1)
+            use = fn;
+        if (ab && k < 2)
+            HIP_TRY(hipEventRecord(r->ev_ab[2 * k], r->stream));
+        int rc;
+        if (use) {
+            const void *pa = args;
+            void *params[] = {&pa};
+            rc = (int)hipModuleLaunchKernel((hipFunction_t)use, (unsigned)p.grid, 1, 1,
+                                            (unsigned)p.block, 1, 1, (unsigned)p.lds, r->stream,
+                                            params, nullptr);
+        } else {
+            WalkLaunch L;
+            L.nc = 0;
+            L.vs = r->vs;
+            L.bin = true;
+            L.grid = p.grid;
+            L.block = p.block;
+            L.lds = p.lds;
+            L.stream = r->stream;
+            L.args = args;
+            rc = launch_walk_any(r->mode, r->spec.dims, L);
+        }
         if (rc)
             return rc;
+        if (ab && k < 2)
+            HIP_TRY(hipEventRecord(r->ev_ab[2 * k + 1], r->stream));
+        if (ab && k == 1) {
+            float t_table = 0, t_compiled = 0;
+            HIP_TRY(hipEventSynchronize(r->ev_ab[3]));
+            HIP_TRY(hipEventElapsedTime(&t_table, r->ev_ab[0], r->ev_ab[1]));
+            HIP_TRY(hipEventElapsedTime(&t_compiled, r->ev_ab[2], r->ev_ab[3]));
+            const bool faster = t_compiled < t_table;
+            r->compiled_choice[key] = faster;
+            use = faster ? fn : nullptr;
+            ab = false;
+        }
         rc = launch_bin_accumulate(r->bin, n_rec, r->cells, r->dbuf, (int)r->word_bytes,
                                    r->stream);
         if (rc)
             return rc;
     }
+    r->last_compiled = use != nullptr;
     return 0;
 }
 
@@ -1066,8 +1136,8 @@ static int launch(ffr_renderer *r, uint64_t samples)
     HIP_TRY(hipEventRecord(r->ev0, r->stream));
     if (p.bin)
         rc = r->mode == MODE_F64_U64
-                 ? launch_binned(r, r->f64, p, r->next_walker, walkers, batch, last)
-                 : launch_binned(r, r->f32, p, r->next_walker, walkers, batch, last);
+                 ? launch_binned(r, r->f64, p, r->next_walker, walkers, batch, last, samples)
+                 : launch_binned(r, r->f32, p, r->next_walker, walkers, batch, last, samples);
     else
         rc = r->mode == MODE_F64_U64
                  ? launch_single(r, r->f64, p, r->next_walker, walkers, batch, last)

@@ -112,6 +112,16 @@ const char *mode_types(int mode)
     }
 }
 
+// trajectory and count types of the binned walker (count-only buffers)
+const char *bin_types(int mode)
+{
+    switch (mode) {
+    case MODE_F64_U64: return "double, unsigned long long";
+    case MODE_F32_U32: return "float, unsigned int";
+    default: return "float, unsigned long long";
+    }
+}
+
 template <typename R>
 std::string flame_source(const WalkArgs<R> &a, const std::vector<R> &cw, const RtcKernel &k)
 {
@@ -156,10 +166,13 @@ std::string flame_source(const WalkArgs<R> &a, const std::vector<R> &cw, const R
     s << "};\n} // namespace ffr\n";
     s << "extern \"C\" __global__ void __launch_bounds__(" << k.block_max
       << ") ffr_flame_walk(const ffr::FlameArgs *__restrict__ args)\n{\n"
-         "    extern __shared__ __align__(16) unsigned char smem[];\n"
-         "    ffr::walk_tasks<"
-      << mode_types(k.mode) << ", " << k.dims << ", " << k.nc << ", " << k.vs << ", "
-      << (k.cache ? "true" : "false") << ">(*args, smem);\n}\n";
+         "    extern __shared__ __align__(16) unsigned char smem[];\n";
+    if (k.bin)
+        s << "    ffr::walk_bin_tasks<" << bin_types(k.mode) << ", " << k.dims << ", " << k.vs
+          << ">(*args, smem);\n}\n";
+    else
+        s << "    ffr::walk_tasks<" << mode_types(k.mode) << ", " << k.dims << ", " << k.nc << ", "
+          << k.vs << ", " << (k.cache ? "true" : "false") << ">(*args, smem);\n}\n";
     return s.str();
 }
 

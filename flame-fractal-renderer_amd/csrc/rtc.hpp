@@ -15,6 +15,7 @@ struct RtcKernel {
     int mode = 0, dims = 2, nc = 0, vs = 0;
     bool cache = false;
     int block_max = 256;
+    bool bin = false; // a pass of binned accumulation (walk_bin_tasks)
 };
 
 // the flame's kernel (a hipFunction_t) for this flame (its arguments and

@@ -76,10 +76,12 @@ typedef struct ffr_options {
      * reference, whose flame is objects with virtual calls): the walker code
      * built at run time with hiprtc, the flame's xform count, table layout,
      * variation types and bounds as constants, kept for the process, for
-     * renders that take the lane-per-walker kernel (-1, the default: chosen
-     * by a timed probe of both kernels, on a scratch buffer, once a renderer
-     * renders again after its renders have asked for 2^33 samples in all;
-     * 0 off; 1 on).  Results equal the table-driven kernel's bit for bit. */
+     * renders that take the lane-per-walker kernel or binned accumulation
+     * (-1, the default: chosen by timing both kernels once a renderer
+     * renders again after its renders have asked for 2^33 samples in all,
+     * on a scratch buffer for the lane kernel, on the render's first two
+     * passes for the binned walker; 0 off; 1 on).  Results equal the
+     * table-driven kernel's bit for bit. */
     int32_t compiled;
     int32_t reserved;
 } ffr_options;

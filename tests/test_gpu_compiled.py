@@ -137,17 +137,96 @@ def test_compiled_linear_flames_match_oracle(ffr_mod, oracle_mod, name, mode):
     r.close()
 
 
-def test_compiled_not_taken_by_other_kernels(ffr_mod):
-    """the wave-per-walker and binned paths keep their own kernels"""
+def test_compiled_not_taken_by_the_wave_per_walker_kernel(ffr_mod):
+    """the wave-per-walker path keeps its own kernel"""
     sier = read_flame("sierpinski_1024.json").replace("[1024, 1024]", "[64, 64]")
     r, ok = render(ffr_mod, sier, 64_000, 1000, "fp64", 1)
     assert ok and r.last_stats.wave_per_walker == 1 and r.last_stats.compiled == 0
     r.close()
-    r = ffr_mod.BufferRenderer(TF.tetrahedron(size=64), seed=1, batch_size=500, binned=1,
-                               compiled=1)
-    assert r.render(100_000)
-    assert r.last_stats.binned == 1 and r.last_stats.compiled == 0
-    r.close()
+
+
+@pytest.fixture
+def small_passes():
+    """passes of 2^15 records: the renders below take tens of passes"""
+    old = os.environ.get("FFR_BIN_PASS_RECORDS")
+    os.environ["FFR_BIN_PASS_RECORDS"] = str(1 << 15)
+    yield
+    if old is None:
+        del os.environ["FFR_BIN_PASS_RECORDS"]
+    else:
+        os.environ["FFR_BIN_PASS_RECORDS"] = old
+
+
+def without_colour(text):
+    """the flame with its colours removed (a count-only buffer)"""
+    d = json.loads(text)
+    d.pop("color_dimensions", None)
+    for x in d["xforms"]:
+        x.pop("color", None)
+    return json.dumps(d)
+
+
+def binned_render(ffr, text, samples, batch, mode, compiled):
+    precision = "fp64" if mode == "fp64" else "fp32"
+    word = 32 if mode == "fp32" else 64
+    r = ffr.BufferRenderer(text, precision=precision, word=word, seed=SEED, batch_size=batch,
+                           settle=20, binned=1, compiled=compiled)
+    return r, r.render(samples)
+
+
+BINNED = {
+    # name: (maker, dims, samples, batch)
+    "tetrahedron": (lambda: TF.tetrahedron(size=64), 3, 600_000, 1500),
+    "fern": (lambda: TF.fern(size=300), 2, 500_000, 999),
+    "catalogue_c": (lambda: without_colour(catalogue(5)), 2, 300_000, 300),
+}
+
+
+@pytest.mark.parametrize("name,mode", [("tetrahedron", "fp32"), ("tetrahedron", "fp64"),
+                                       ("fern", "fp32u64"), ("catalogue_c", "fp64")])
+def test_compiled_binned_walker(ffr_mod, oracle_mod, small_passes, name, mode):
+    """binned accumulation with the flame's compiled walker: the same buffer
+    and statistics as the table-driven walker, pass after pass; linear
+    flames equal the oracle"""
+    maker, dims, samples, batch = BINNED[name]
+    text = maker()
+    a, oka = binned_render(ffr_mod, text, samples, batch, mode, 1)
+    b, okb = binned_render(ffr_mod, text, samples, batch, mode, 0)
+    assert oka and okb
+    assert a.last_stats.binned == 1 and a.last_stats.compiled == 1
+    assert b.last_stats.binned == 1 and b.last_stats.compiled == 0
+    assert np.array_equal(a.read_buffer(), b.read_buffer())
+    assert a.samples_iterated == b.samples_iterated
+    assert a.samples_plotted == b.samples_plotted
+    assert np.array_equal(a.xform_distribution(), b.xform_distribution())
+    for i in range(dims):
+        assert a.point_extremes[i] == b.point_extremes[i]
+    if name != "catalogue_c":
+        ref = oracle_mod.OracleFlame(text, MODES[mode], settle=20).render(
+            samples, batch, seed=SEED, threads=8)
+        assert np.array_equal(a.read_buffer(), ref.buffer)
+    a.close()
+    b.close()
+
+
+def test_compiled_binned_auto_probe(ffr_mod, small_passes, low_probe_threshold):
+    """compiled -1 on the binned path: a render in steady use times its first
+    pass with the table-driven walker and its second with the compiled one,
+    keeps the faster, and the buffer equals a renderer's that never compiles"""
+    maker, dims, samples, batch = BINNED["tetrahedron"]
+    text = maker()
+    a, oka = binned_render(ffr_mod, text, samples, batch, "fp32", -1)
+    b, okb = binned_render(ffr_mod, text, samples, batch, "fp32", 0)
+    assert oka and okb and a.last_stats.compiled == 0
+    kept = []
+    for _ in range(3):
+        assert a.render(samples) and b.render(samples)
+        kept.append(a.last_stats.compiled)
+        assert np.array_equal(a.read_buffer(), b.read_buffer())
+        assert a.samples_plotted == b.samples_plotted
+    assert kept[0] == kept[1] == kept[2]
+    a.close()
+    b.close()
 
 
 @pytest.fixture
