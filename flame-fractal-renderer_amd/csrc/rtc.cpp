@@ -23,7 +23,9 @@
 #include <chrono>
 #include <cinttypes>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -254,6 +256,16 @@ int rtc_flame_kernel(const WalkArgs<R> &a, const std::vector<R> &cw, const RtcKe
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_cache.find(key);
     if (it == g_cache.end()) {
+        // FFR_RTC_DUMP: a directory that receives each generated source
+        // (to read or rebuild a flame's kernel offline)
+        if (const char *dir = std::getenv("FFR_RTC_DUMP")) {
+            const std::string path = std::string(dir) + "/ffr_flame_" +
+                                     std::to_string(std::hash<std::string>{}(src)) + ".hip";
+            if (FILE *f = std::fopen(path.c_str(), "w")) {
+                std::fputs(src.c_str(), f);
+                std::fclose(f);
+            }
+        }
         Compiled c;
         std::string log;
         std::vector<char> code;

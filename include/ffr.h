@@ -134,7 +134,12 @@ const char *ffr_last_error(void);
 int ffr_render(ffr_renderer *r, uint64_t samples, ffr_stats *stats);
 
 /* enqueue the same work on the renderer's stream without waiting;
- * ffr_sync waits, folds the statistics in and reports like ffr_render */
+ * ffr_sync waits, folds the statistics in and reports like ffr_render.
+ * Not for stream capture as configured by default: the probes of
+ * compiled -1 and cell_cache -1 (once per renderer in steady use) allocate
+ * scratch memory and wait for the stream while they time the kernels, and
+ * binned accumulation allocates its record arrays on first use and copies
+ * its pass arguments from host memory. */
 int ffr_render_async(ffr_renderer *r, uint64_t samples);
 int ffr_sync(ffr_renderer *r, ffr_stats *stats);
 
