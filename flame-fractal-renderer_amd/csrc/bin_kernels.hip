@@ -319,6 +319,183 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
+// Colour flames: the scatter also moves each record's colour.  The chunk's
+// keys are sorted in LDS with their positions in the chunk, and the colour
+// components are copied from there (a chunk's records were just read, so
+// their colours come from the caches).
+constexpr size_t scatter_pay_lds(int ntiles, int per)
+{
+    return sizeof(unsigned) * (2 * (size_t)ntiles + 2 * (size_t)kScatterLanes * (size_t)per);
+}
+
+template <int PER, typename PR>
+__global__ void __launch_bounds__(kScatterLanes)
+    bin_scatter_pay_kernel(const unsigned *rec, const PR *pay, unsigned long long n,
+                           unsigned long long per_block, int tile_bits, int ntiles,
+                           const unsigned *hist, const unsigned *start, unsigned *sorted,
+                           PR *pay_sorted, unsigned long long stride, int nr)
+{
+    extern __shared__ unsigned lds[];
+    unsigned *cur = lds;                          // [ntiles] next slot of each tile
+    unsigned *off = cur + ntiles;                 // [ntiles] chunk counts, then ends
+    unsigned *stage = off + ntiles;               // [chunk] keys in tile order
+    unsigned *src = stage + kScatterLanes * PER;  // [chunk] their places in the chunk
+    unsigned *wsum = stage;                       // [16] wave sums of the scan
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const unsigned *row = hist + (size_t)blockIdx.x * (size_t)ntiles;
+    for (int t = tid; t < ntiles; t += kScatterLanes)
+        cur[t] = start[t] + row[t];
+    const unsigned long long b0 = (unsigned long long)blockIdx.x * per_block;
+    const unsigned long long b1 = b0 + per_block < n ? b0 + per_block : n;
+    constexpr unsigned long long chunk = (unsigned long long)kScatterLanes * PER;
+    const int per_t = (ntiles + kScatterLanes - 1) / kScatterLanes;
+    for (unsigned long long c0 = b0; c0 < b1; c0 += chunk) {
+        for (int t = tid; t < ntiles; t += kScatterLanes)
+            off[t] = 0;
+        __syncthreads();
+        unsigned c[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const unsigned long long i = c0 + (unsigned long long)k * kScatterLanes + tid;
+            c[k] = i < b1 ? rec[i] : kBinEmpty;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (c[k] != kBinEmpty)
+                atomicAdd(&off[c[k] >> tile_bits], 1u);
+        __syncthreads();
+        const int t0 = tid * per_t;
+        unsigned mine = 0;
+        for (int j = 0; j < per_t; ++j)
+            if (t0 + j < ntiles)
+                mine += off[t0 + j];
+        unsigned incl = mine;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(incl, d, 64);
+            if (lane >= d)
+                incl += y;
+        }
+        if (lane == 63)
+            wsum[wave] = incl;
+        __syncthreads();
+        unsigned before = 0, total = 0;
+        for (int w = 0; w < kScatterLanes / 64; ++w) {
+            const unsigned ws = wsum[w];
+            before += w < wave ? ws : 0u;
+            total += ws;
+        }
+        unsigned run = before + incl - mine;
+        for (int j = 0; j < per_t; ++j)
+            if (t0 + j < ntiles) {
+                const unsigned v = off[t0 + j];
+                off[t0 + j] = run;
+                run += v;
+            }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (c[k] != kBinEmpty) {
+                const unsigned at = atomicAdd(&off[c[k] >> tile_bits], 1u);
+                stage[at] = c[k];
+                src[at] = (unsigned)(k * kScatterLanes + tid);
+            }
+        __syncthreads();
+        for (unsigned i = (unsigned)tid; i < total; i += kScatterLanes) {
+            const unsigned r = stage[i];
+            const unsigned t = r >> tile_bits;
+            const unsigned long long g = cur[t] + (i - (t ? off[t - 1] : 0u));
+            sorted[g] = r;
+            const unsigned long long from = c0 + src[i];
+            for (int j = 0; j < nr; ++j)
+                pay_sorted[(unsigned long long)j * stride + g] =
+                    pay[(unsigned long long)j * stride + from];
+        }
+        __syncthreads();
+        for (int t = tid; t < ntiles; t += kScatterLanes)
+            cur[t] += off[t] - (t ? off[t - 1] : 0u);
+        __syncthreads();
+    }
+}
+
+// Colour flames: a chunk's counts and colour sums in LDS (u32 counts, f64
+// sums of the records' colours), then one add per word of each occupied
+// cell: plain where the tile has one chunk, atomic where chunks share it.
+// Colour sums come out in another order than the atomic path's per-sample
+// adds, equal to the rounding of that order.
+constexpr int kColourTile = 1 << kBinTileBitsColour;
+
+template <typename CT, typename CR, typename PR>
+__global__ void __launch_bounds__(1024)
+    bin_accum_colour_kernel(const unsigned *sorted, const PR *pay_sorted,
+                            unsigned long long stride, int nr, const unsigned *start,
+                            const unsigned *cstart, int ntiles, unsigned long long cells,
+                            CT *buf)
+{
+    extern __shared__ unsigned long long lds64[];
+    double *col = reinterpret_cast<double *>(lds64);                 // [nr][kColourTile]
+    unsigned *cnt = reinterpret_cast<unsigned *>(col + 4 * kColourTile); // [kColourTile]
+    const unsigned w = blockIdx.x;
+    if (w >= cstart[ntiles])
+        return;
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cstart[mid] <= w)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int t = lo;
+    const unsigned k = w - cstart[t];
+    const bool shared = cstart[t + 1] - cstart[t] > 1;
+    const unsigned s0 = start[t] + k * kAccumChunk;
+    const unsigned s1 = min(start[t + 1], s0 + kAccumChunk);
+    const unsigned long long c0 = (unsigned long long)t * kColourTile;
+    const unsigned long long left = cells - c0;
+    const unsigned ncell = left < (unsigned long long)kColourTile ? (unsigned)left : kColourTile;
+    const unsigned nzero = (ncell + 31u) & ~31u;
+    for (unsigned i = threadIdx.x; i < nzero; i += blockDim.x) {
+        cnt[i] = 0;
+        for (int j = 0; j < nr; ++j)
+            col[j * kColourTile + i] = 0.0;
+    }
+    __syncthreads();
+    constexpr unsigned mask = kColourTile - 1;
+    for (unsigned i = s0 + threadIdx.x; i < s1; i += blockDim.x) {
+        const unsigned cw = tile_word(sorted[i] & mask);
+        atomicAdd(&cnt[cw], 1u);
+        for (int j = 0; j < nr; ++j)
+            atomicAdd(&col[j * kColourTile + cw],
+                      (double)pay_sorted[(unsigned long long)j * stride + i]);
+    }
+    __syncthreads();
+    const int words = 1 + nr;
+    for (unsigned i = threadIdx.x; i < ncell; i += blockDim.x) {
+        const unsigned cw = tile_word(i);
+        const unsigned v = cnt[cw];
+        if (!v)
+            continue;
+        CT *cell = buf + (c0 + i) * (unsigned long long)words;
+        CR *colour = reinterpret_cast<CR *>(cell + 1);
+        if (shared) {
+            atomicAdd(cell, (CT)v);
+            for (int j = 0; j < nr; ++j)
+                unsafeAtomicAdd(&colour[j], (CR)col[j * kColourTile + cw]);
+        } else {
+            *cell += (CT)v;
+            for (int j = 0; j < nr; ++j)
+                colour[j] += (CR)col[j * kColourTile + cw];
+        }
+    }
+}
+
+constexpr size_t accum_colour_lds()
+{
+    return sizeof(double) * 4 * kColourTile + sizeof(unsigned) * kColourTile;
+}
+
 int grid_for(unsigned long long n, int block)
 {
     unsigned long long g = (n + (unsigned long long)block - 1) / (unsigned long long)block;
@@ -334,43 +511,91 @@ size_t bin_scratch_hist_words(unsigned long long records, int ntiles)
 }
 
 int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,
-                          unsigned long long cells, void *buf, int word_bytes, hipStream_t s)
+                          unsigned long long cells, void *buf, int word_bytes, int r,
+                          int real_bytes, hipStream_t s)
 {
-    const int ntiles = (int)((cells + (1ull << kBinTileBits) - 1) >> kBinTileBits);
-    if (ntiles > kBinMaxTiles || n_records > 0xFFFFFFFFull)
+    const int tile_bits = r > 0 ? kBinTileBitsColour : kBinTileBits;
+    const int ntiles = (int)((cells + (1ull << tile_bits) - 1) >> tile_bits);
+    if (ntiles > kBinMaxTiles || n_records > 0xFFFFFFFFull || r < 0 || r > 4 ||
+        (r > 0 && (real_bytes != 4 && real_bytes != 8)))
         return (int)hipErrorInvalidValue;
     const unsigned long long per = kBinRecordsPerBlock;
     const int nblocks = (int)std::max<unsigned long long>(1, (n_records + per - 1) / per);
     const size_t hist_lds = sizeof(unsigned) * (size_t)ntiles;
     hipLaunchKernelGGL(bin_hist_kernel, dim3(nblocks), dim3(kBinBlock), hist_lds, s, b.rec,
-                       n_records, per, kBinTileBits, ntiles, b.hist);
+                       n_records, per, tile_bits, ntiles, b.hist);
     hipLaunchKernelGGL(bin_colscan_kernel, dim3((ntiles + kBinBlock - 1) / kBinBlock),
                        dim3(kBinBlock), 0, s, b.hist, nblocks, ntiles, b.total);
     hipLaunchKernelGGL(bin_tilescan_kernel, dim3(1), dim3(1024), 0, s, b.total, ntiles, b.start);
-    if (ntiles <= 4096)
-        hipLaunchKernelGGL(bin_scatter_kernel<32>, dim3(nblocks), dim3(kScatterLanes),
-                           scatter_lds(ntiles, 32), s, b.rec, n_records, per, kBinTileBits, ntiles,
-                           b.hist, b.start, b.sorted);
-    else
-        hipLaunchKernelGGL(bin_scatter_kernel<16>, dim3(nblocks), dim3(kScatterLanes),
-                           scatter_lds(ntiles, 16), s, b.rec, n_records, per, kBinTileBits, ntiles,
-                           b.hist, b.start, b.sorted);
+    const unsigned long long stride = b.capacity;
+    if (r == 0) {
+        if (ntiles <= 4096)
+            hipLaunchKernelGGL(bin_scatter_kernel<32>, dim3(nblocks), dim3(kScatterLanes),
+                               scatter_lds(ntiles, 32), s, b.rec, n_records, per, tile_bits,
+                               ntiles, b.hist, b.start, b.sorted);
+        else
+            hipLaunchKernelGGL(bin_scatter_kernel<16>, dim3(nblocks), dim3(kScatterLanes),
+                               scatter_lds(ntiles, 16), s, b.rec, n_records, per, tile_bits,
+                               ntiles, b.hist, b.start, b.sorted);
+    } else if (real_bytes == 8) {
+        if (ntiles <= 2048)
+            hipLaunchKernelGGL((bin_scatter_pay_kernel<16, double>), dim3(nblocks),
+                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 16), s, b.rec,
+                               (const double *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
+                               b.start, b.sorted, (double *)b.pay_sorted, stride, r);
+        else
+            hipLaunchKernelGGL((bin_scatter_pay_kernel<8, double>), dim3(nblocks),
+                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 8), s, b.rec,
+                               (const double *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
+                               b.start, b.sorted, (double *)b.pay_sorted, stride, r);
+    } else {
+        if (ntiles <= 2048)
+            hipLaunchKernelGGL((bin_scatter_pay_kernel<16, float>), dim3(nblocks),
+                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 16), s, b.rec,
+                               (const float *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
+                               b.start, b.sorted, (float *)b.pay_sorted, stride, r);
+        else
+            hipLaunchKernelGGL((bin_scatter_pay_kernel<8, float>), dim3(nblocks),
+                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 8), s, b.rec,
+                               (const float *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
+                               b.start, b.sorted, (float *)b.pay_sorted, stride, r);
+    }
     // accumulation work items: per tile, chunks of at most kAccumChunk
     // records (at most ntiles + n_records / kAccumChunk items in all)
     unsigned *chunks = b.total, *cstart = b.total + ntiles + 1;
     hipLaunchKernelGGL(bin_chunks_kernel, dim3((ntiles + kBinBlock - 1) / kBinBlock),
                        dim3(kBinBlock), 0, s, b.start, ntiles, chunks);
     hipLaunchKernelGGL(bin_tilescan_kernel, dim3(1), dim3(1024), 0, s, chunks, ntiles, cstart);
-    const unsigned long long items = (unsigned long long)ntiles + n_records / kAccumChunk;
-    const size_t tile_lds = sizeof(unsigned) << kBinTileBits;
-    if (word_bytes == 8)
-        hipLaunchKernelGGL((bin_accum_kernel<unsigned long long>), dim3((unsigned)items),
-                           dim3(1024), tile_lds, s, b.sorted, b.start, cstart, ntiles,
-                           kBinTileBits, cells, (unsigned long long *)buf);
+    const unsigned items = (unsigned)((unsigned long long)ntiles + n_records / kAccumChunk);
+    if (r == 0) {
+        const size_t tile_lds = sizeof(unsigned) << kBinTileBits;
+        if (word_bytes == 8)
+            hipLaunchKernelGGL((bin_accum_kernel<unsigned long long>), dim3(items), dim3(1024),
+                               tile_lds, s, b.sorted, b.start, cstart, ntiles, tile_bits, cells,
+                               (unsigned long long *)buf);
+        else
+            hipLaunchKernelGGL((bin_accum_kernel<unsigned>), dim3(items), dim3(1024), tile_lds,
+                               s, b.sorted, b.start, cstart, ntiles, tile_bits, cells,
+                               (unsigned *)buf);
+        return (int)hipGetLastError();
+    }
+    // colour: 8-byte words hold u64 counts and f64 sums, 4-byte words u32
+    // and f32 (the float build); the records' colours are the walker's reals
+    if (word_bytes == 8 && real_bytes == 8)
+        hipLaunchKernelGGL((bin_accum_colour_kernel<unsigned long long, double, double>),
+                           dim3(items), dim3(1024), accum_colour_lds(), s, b.sorted,
+                           (const double *)b.pay_sorted, stride, r, b.start, cstart, ntiles,
+                           cells, (unsigned long long *)buf);
+    else if (word_bytes == 8)
+        hipLaunchKernelGGL((bin_accum_colour_kernel<unsigned long long, double, float>),
+                           dim3(items), dim3(1024), accum_colour_lds(), s, b.sorted,
+                           (const float *)b.pay_sorted, stride, r, b.start, cstart, ntiles,
+                           cells, (unsigned long long *)buf);
     else
-        hipLaunchKernelGGL((bin_accum_kernel<unsigned>), dim3((unsigned)items), dim3(1024),
-                           tile_lds, s, b.sorted, b.start, cstart, ntiles, kBinTileBits, cells,
-                           (unsigned *)buf);
+        hipLaunchKernelGGL((bin_accum_colour_kernel<unsigned, float, float>), dim3(items),
+                           dim3(1024), accum_colour_lds(), s, b.sorted,
+                           (const float *)b.pay_sorted, stride, r, b.start, cstart, ntiles,
+                           cells, (unsigned *)buf);
     return (int)hipGetLastError();
 }
 
@@ -386,6 +611,9 @@ int prepare_bin_kernels()
     // the scatter kernel: chunks of 32 records a lane up to 4096 tiles, of
     // 16 up to 8192
     static_assert(scatter_lds(4096, 32) <= 160 * 1024, "scatter chunk beyond the LDS");
+    static_assert(scatter_pay_lds(2048, 16) <= 160 * 1024, "scatter chunk beyond the LDS");
+    static_assert(scatter_pay_lds(kBinMaxTiles, 8) <= 160 * 1024, "scatter chunk beyond the LDS");
+    static_assert(accum_colour_lds() <= 160 * 1024, "colour tile beyond the LDS");
     static_assert(scatter_lds(kBinMaxTiles, 16) <= 160 * 1024, "scatter chunk beyond the LDS");
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void *)bin_scatter_kernel<32>,
@@ -395,6 +623,28 @@ int prepare_bin_kernels()
         e = hipFuncSetAttribute((const void *)bin_scatter_kernel<16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)scatter_lds(kBinMaxTiles, 16));
+    // colour flames: chunks of 16 records a lane up to 2048 tiles, of 8 up
+    // to 8192; the accumulation's tile of counts and four f64 sums
+    const void *pay16[] = {(const void *)bin_scatter_pay_kernel<16, double>,
+                           (const void *)bin_scatter_pay_kernel<16, float>};
+    const void *pay8[] = {(const void *)bin_scatter_pay_kernel<8, double>,
+                          (const void *)bin_scatter_pay_kernel<8, float>};
+    for (const void *k : pay16)
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)scatter_pay_lds(2048, 16));
+    for (const void *k : pay8)
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)scatter_pay_lds(kBinMaxTiles, 8));
+    const void *acc[] = {
+        (const void *)bin_accum_colour_kernel<unsigned long long, double, double>,
+        (const void *)bin_accum_colour_kernel<unsigned long long, double, float>,
+        (const void *)bin_accum_colour_kernel<unsigned, float, float>};
+    for (const void *k : acc)
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)accum_colour_lds());
     return (int)e;
 }
 

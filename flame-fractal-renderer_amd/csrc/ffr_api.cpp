@@ -44,6 +44,12 @@ int fail(int code, const std::string &msg)
             return fail(FFR_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Colour flames switch to binned accumulation after a render of at least
+// kBinColourMinSamples samples made kBinColourAtomics or more global atomic
+// adds (LDS-cache misses, a count and each colour word) per sample.
+constexpr uint64_t kBinColourMinSamples = 1ull << 26;
+constexpr double kBinColourAtomics = 1.0;
+
 struct DevTables {
     void *table = nullptr;
     unsigned long long *masks = nullptr;
@@ -127,6 +133,11 @@ struct ffr_renderer {
     // arguments are allocated on the first binned render and kept.
     int bin_pref = -1;
     BinScratch bin;
+    size_t bin_pay_bytes = 0; // bytes of bin.pay (and of bin.pay_sorted)
+    // a colour flame whose renders made at least kBinColourAtomics global
+    // atomic adds per sample: its later single renders take binned
+    // accumulation (binned -1)
+    bool bin_colour = false;
     size_t bin_hist_words = 0;
     size_t bin_tiles = 0;
     void *wstate = nullptr;
@@ -408,6 +419,8 @@ void ffr_destroy(ffr_renderer *r)
     (void)hipFree(r->bin.hist);
     (void)hipFree(r->bin.total);
     (void)hipFree(r->bin.start);
+    (void)hipFree(r->bin.pay);
+    (void)hipFree(r->bin.pay_sorted);
     (void)hipFree(r->wstate);
     (void)hipFree(r->dargs_bin);
     for (int k = 0; k < 2; ++k) {
@@ -507,11 +520,13 @@ struct LaunchShape {
     int cache_kb = 144; // LDS budget of the cache per workgroup, KiB
     int block = 0;      // lanes per workgroup, 0 chosen
     int task_rounds = 1; // walkers per lane in one workgroup task
+    int bin = -1;       // binned accumulation (single renders): -1 chosen, 0 off, 1 on
 };
 
 static LaunchShape renderer_shape(const ffr_renderer *r)
 {
     LaunchShape sh;
+    sh.bin = r->bin_pref;
     sh.cache = r->cache_pref;
     sh.cache_kb = r->cache_kb;
     sh.block = r->block_pref;
@@ -552,16 +567,21 @@ static LaunchPlan plan_launch(const ffr_renderer *r, uint64_t walkers, uint64_t 
                            : chain_walkers_per_cu(r->chain_post_id, r->spec.color_dims);
     const bool chain = sh.cache < 0 && sh.block <= 0 && base_lds == 0 && r->chain_ok &&
                        walkers <= (uint64_t)r->cu_count * (uint64_t)per_cu;
-    // binned accumulation: single renders of count-only buffers under 2^28
-    // cells whose walkers all fit one pass's records; chosen from 2^22 cells
-    // on when the lane-per-walker kernel runs (the samples spread beyond
-    // what an LDS cache holds), or when asked for
+    // binned accumulation: single renders whose walkers all fit one pass's
+    // records, of count-only buffers under 2^28 cells or colour buffers (up
+    // to four colour dimensions) under 2^25; chosen for count-only buffers
+    // from 2^22 cells on when the lane-per-walker kernel runs (the samples
+    // spread beyond what an LDS cache holds), for colour flames whose
+    // earlier renders spent many global atomics per sample (bin_colour),
+    // or when asked for
     const uint64_t tw_bin = (uint64_t)kBinWalkBlock;
-    const bool bin_ok = base_lds == 0 && r->spec.color_dims == 0 && r->cells < (1ull << 28) &&
+    const int ncol = r->spec.color_dims;
+    const bool bin_ok = base_lds == 0 && ncol <= 4 &&
+                        r->cells < (ncol > 0 ? (1ull << 25) : (1ull << 28)) &&
                         (walkers + tw_bin - 1) / tw_bin * tw_bin <= kBinPassRecords;
-    if (bin_ok && (r->bin_pref == 1 ||
-                   (r->bin_pref < 0 && !chain && sh.cache < 0 && sh.block <= 0 &&
-                    r->cells >= (1ull << 22)))) {
+    const bool bin_auto = sh.bin < 0 && !chain && sh.cache < 0 && sh.block <= 0 &&
+                          (ncol == 0 ? r->cells >= (1ull << 22) : r->bin_colour);
+    if (bin_ok && (sh.bin == 1 || bin_auto)) {
         p.bin = true;
         p.block = kBinWalkBlock;
         p.task_walkers = tw_bin;
@@ -964,7 +984,9 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
     const uint64_t span = std::max<uint64_t>(1, std::min<uint64_t>(batch, pass_records / slots));
     const uint64_t npass = (batch + span - 1) / span;
     const uint64_t cap = slots * span;
-    const size_t ntiles = (size_t)((r->cells + (1ull << kBinTileBits) - 1) >> kBinTileBits);
+    const int ncol = r->spec.color_dims; // colour components of each record
+    const int tile_bits = ncol > 0 ? kBinTileBitsColour : kBinTileBits;
+    const size_t ntiles = (size_t)((r->cells + (1ull << tile_bits) - 1) >> tile_bits);
     hipError_t e = hipSuccess;
     size_t have_rec = (size_t)r->bin.capacity * 4, have_sorted = have_rec;
     if (have_rec < cap * 4) {
@@ -972,6 +994,16 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
         if (e == hipSuccess)
             e = ensure_device((void **)&r->bin.sorted, have_sorted, (size_t)cap * 4);
         r->bin.capacity = e == hipSuccess ? cap : 0;
+    }
+    // colour flames: the records' colours, component i at i * capacity
+    if (e == hipSuccess && ncol > 0) {
+        const size_t need = (size_t)ncol * (size_t)r->bin.capacity * sizeof(R);
+        size_t have_sorted_pay = r->bin_pay_bytes;
+        e = ensure_device(&r->bin.pay, r->bin_pay_bytes, need);
+        if (e == hipSuccess)
+            e = ensure_device(&r->bin.pay_sorted, have_sorted_pay, need);
+        if (e != hipSuccess)
+            r->bin_pay_bytes = 0;
     }
     size_t hist_bytes = r->bin_hist_words * 4;
     if (e == hipSuccess)
@@ -1004,6 +1036,8 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
         walk_args(r, d, p, walker0, walkers, batch, last, a);
         a.rec = r->bin.rec;
         a.wstate = r->wstate;
+        a.pay = r->bin.pay;
+        a.pay_stride = r->bin.capacity;
         a.it0 = k * span;
         a.it1 = std::min<uint64_t>(batch, a.it0 + span);
         std::memcpy(r->hargs.data() + k * sizeof a, &a, sizeof a);
@@ -1035,6 +1069,7 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
             RtcKernel k;
             k.mode = r->mode;
             k.dims = r->spec.dims;
+            k.nc = ncol;
             k.vs = r->vs;
             k.block_max = kBinWalkBlock;
             k.bin = true;
@@ -1066,7 +1101,7 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
                                             params, nullptr);
         } else {
             WalkLaunch L;
-            L.nc = 0;
+            L.nc = ncol;
             L.vs = r->vs;
             L.bin = true;
             L.grid = p.grid;
@@ -1090,8 +1125,8 @@ static int launch_binned(ffr_renderer *r, const DeviceFlame<R> &d, const LaunchP
             use = faster ? fn : nullptr;
             ab = false;
         }
-        rc = launch_bin_accumulate(r->bin, n_rec, r->cells, r->dbuf, (int)r->word_bytes,
-                                   r->stream);
+        rc = launch_bin_accumulate(r->bin, n_rec, r->cells, r->dbuf, (int)r->word_bytes, ncol,
+                                   (int)sizeof(R), r->stream);
         if (rc)
             return rc;
     }
@@ -1180,6 +1215,13 @@ static int finish(ffr_renderer *r, ffr_stats *out)
         r->last_bad = st.n_bad;
         r->last_atomics = st.atomics;
         r->last_flush_atomics = st.flush_atomics;
+        // a colour flame whose samples missed the LDS cache often enough
+        // renders faster binned from now on (its global atomics cost more
+        // than the records' trips through HBM)
+        if (r->spec.color_dims > 0 && !r->last_bin && !r->last_chain &&
+            st.s_iter >= kBinColourMinSamples &&
+            (double)(st.atomics - st.flush_atomics) >= kBinColourAtomics * (double)st.s_iter)
+            r->bin_colour = true;
         size_t nrec = (size_t)std::min<uint64_t>(st.n_bad, r->opt.bad_value_cap);
         r->bv_ids.assign(nrec, 0);
         r->bv_pts.assign(nrec * 3, 0.0);

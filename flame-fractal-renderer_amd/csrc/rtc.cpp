@@ -170,8 +170,8 @@ std::string flame_source(const WalkArgs<R> &a, const std::vector<R> &cw, const R
       << ") ffr_flame_walk(const ffr::FlameArgs *__restrict__ args)\n{\n"
          "    extern __shared__ __align__(16) unsigned char smem[];\n";
     if (k.bin)
-        s << "    ffr::walk_bin_tasks<" << bin_types(k.mode) << ", " << k.dims << ", " << k.vs
-          << ">(*args, smem);\n}\n";
+        s << "    ffr::walk_bin_tasks<" << bin_types(k.mode) << ", " << k.dims << ", "
+          << (k.nc > 0 ? 4 : 0) << ", " << k.vs << ">(*args, smem);\n}\n";
     else
         s << "    ffr::walk_tasks<" << mode_types(k.mode) << ", " << k.dims << ", " << k.nc << ", "
           << k.vs << ", " << (k.cache ? "true" : "false") << ">(*args, smem);\n}\n";

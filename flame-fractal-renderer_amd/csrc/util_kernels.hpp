@@ -34,6 +34,10 @@ int launch_fill_zero(void *p, unsigned long long bytes, hipStream_t s);
 // from LDS.  A tile is 2^kBinTileBits cells; at most kBinMaxTiles
 // tiles (cells < 2^28).
 constexpr int kBinTileBits = 15;
+// colour flames: tiles of 2^12 cells, whose u32 counts and up to four f64
+// colour sums (36 B per cell) fit the LDS; at most kBinMaxTiles tiles
+// (cells < 2^25)
+constexpr int kBinTileBitsColour = 12;
 constexpr int kBinMaxTiles = 8192;
 constexpr unsigned long long kBinRecordsPerBlock = 1ull << 18;
 // records of one pass (1 GiB; the sorted copy takes as much again)
@@ -46,12 +50,19 @@ struct BinScratch {
                                  // their first work items (2 (ntiles + 1) words)
     unsigned *start = nullptr;   // per tile: first slot of its region (ntiles + 1)
     unsigned long long capacity = 0; // records rec and sorted hold
+    // colour flames: each record's colour, component i at i * capacity
+    // reals (float or double, as the walker's), and the same grouped by tile
+    void *pay = nullptr;
+    void *pay_sorted = nullptr;
 };
 size_t bin_scratch_hist_words(unsigned long long records, int ntiles);
 // sort the n_records records of b.rec by tile and add the counts into the
 // buffer (cells count words of word_bytes, no colour words)
+// (colour flames: r colour words per cell, the records' colours in b.pay as
+// reals of real_bytes; tile_bits kBinTileBitsColour)
 int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,
-                          unsigned long long cells, void *buf, int word_bytes, hipStream_t s);
+                          unsigned long long cells, void *buf, int word_bytes, int r,
+                          int real_bytes, hipStream_t s);
 // set the accumulate kernel's LDS limit (outside any stream capture)
 int prepare_bin_kernels();
 

@@ -66,6 +66,10 @@ template <typename R> struct WalkDyn {
     unsigned *rec;
     void *wstate;
     uint64_t it0, it1;
+    // colour flames: the plotted samples' colours, component i of record k
+    // at (R *)pay + i * pay_stride + k
+    void *pay;
+    uint64_t pay_stride;
 };
 
 template <typename R> struct WalkArgs : WalkDyn<R> {
@@ -99,6 +103,7 @@ constexpr int kBinWalkBlock = 256;
 // a walker between two passes of binned accumulation
 template <typename R> struct WalkerState {
     R p[3];
+    R c[4];     // colour (colour flames)
     uint64_t n; // stream position, or kStoppedWalker
 };
 constexpr uint64_t kStoppedWalker = ~0ull;

@@ -64,13 +64,17 @@ typedef struct ffr_options {
     int32_t cell_cache;
     int32_t cache_kib;
     int32_t block;
-    /* binned accumulation of count-only buffers: every plotted sample's cell
-     * goes to a record in device memory instead of an atomic add, the
-     * records are sorted by tile of 32768 cells and each tile's counts are
-     * added from LDS (-1 chosen per render: on for flames without colour
-     * whose buffer has at least 2^22 and under 2^28 cells when the
-     * lane-per-walker kernel runs; 0 off; 1 on where the flame allows it:
-     * no colour, under 2^28 cells).  The counts are the same either way. */
+    /* binned accumulation: every plotted sample's cell (and colour) goes to
+     * a record in device memory instead of atomic adds, the records are
+     * sorted by tile (32768 cells; 4096 for colour flames) and each tile's
+     * counts and colour sums are added from LDS (-1 chosen per render: on
+     * for flames without colour whose buffer has at least 2^22 and under
+     * 2^28 cells when the lane-per-walker kernel runs, and for a colour
+     * flame once a render of 2^26 samples or more made at least one global
+     * atomic add per sample; 0 off; 1 on where the flame allows it: under
+     * 2^28 cells without colour, under 2^25 with up to four colours).
+     * Counts are the same either way; colour sums differ by the rounding
+     * of their order of addition. */
     int32_t binned;
     /* the flame compiled into its own kernel (no counterpart in the
      * reference, whose flame is objects with virtual calls): the walker code

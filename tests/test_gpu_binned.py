@@ -125,7 +125,8 @@ def test_binned_bad_value_limit_stops(ffr_mod, small_passes):
 
 def test_binned_is_chosen_for_large_count_buffers(ffr_mod):
     """chosen by default for count-only buffers of 2^22 cells and more when
-    the lane-per-walker kernel runs; not for colour flames or small buffers"""
+    the lane-per-walker kernel runs; not for small buffers, nor for a colour
+    flame's first render; a colour flame takes it when asked for"""
     big = TF.tetrahedron(size=256)              # 2^24 cells
     # 32768 walkers: beyond what the wave-per-walker kernel takes
     r, ok = render(ffr_mod, big, 1 << 24, 512, "fp32", -1)
@@ -136,8 +137,11 @@ def test_binned_is_chosen_for_large_count_buffers(ffr_mod):
     assert ok and r.last_stats.binned == 0
     r.close()
     colour = TF.color_flame(3, size=2048)
-    r, ok = render(ffr_mod, colour, 1 << 22, 2048, "fp64", 1)
+    r, ok = render(ffr_mod, colour, 1 << 22, 2048, "fp64", -1)
     assert ok and r.last_stats.binned == 0
+    r.close()
+    r, ok = render(ffr_mod, colour, 1 << 22, 2048, "fp64", 1)
+    assert ok and r.last_stats.binned == 1
     r.close()
 
 
