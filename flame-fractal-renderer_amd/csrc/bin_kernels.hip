@@ -320,12 +320,15 @@ __global__ void __launch_bounds__(1024)
 }
 
 // Colour flames: the scatter also moves each record's colour.  The chunk's
-// keys are sorted in LDS with their positions in the chunk, and the colour
-// components are copied from there (a chunk's records were just read, so
-// their colours come from the caches).
-constexpr size_t scatter_pay_lds(int ntiles, int per)
+// keys are sorted in LDS; then, one colour component at a time, the chunk's
+// values are read in order (coalesced), placed in LDS where their keys went
+// and written out in tile order.  (Gathered from HBM through the keys'
+// places in the chunk instead, the reads fetched about ten times the
+// colours' bytes.)
+constexpr size_t scatter_pay_lds(int ntiles, int per, int real_bytes)
 {
-    return sizeof(unsigned) * (2 * (size_t)ntiles + 2 * (size_t)kScatterLanes * (size_t)per);
+    return sizeof(unsigned) * (2 * (size_t)ntiles + (size_t)kScatterLanes * (size_t)per) +
+           (size_t)real_bytes * (size_t)kScatterLanes * (size_t)per;
 }
 
 template <int PER, typename PR>
@@ -339,7 +342,9 @@ __global__ void __launch_bounds__(kScatterLanes)
     unsigned *cur = lds;                          // [ntiles] next slot of each tile
     unsigned *off = cur + ntiles;                 // [ntiles] chunk counts, then ends
     unsigned *stage = off + ntiles;               // [chunk] keys in tile order
-    unsigned *src = stage + kScatterLanes * PER;  // [chunk] their places in the chunk
+    // [chunk] one colour component in tile order (8-byte aligned: the
+    // arrays before it are even numbers of words)
+    PR *pl = reinterpret_cast<PR *>(stage + kScatterLanes * PER);
     unsigned *wsum = stage;                       // [16] wave sums of the scan
     const int tid = (int)threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -394,23 +399,35 @@ __global__ void __launch_bounds__(kScatterLanes)
                 run += v;
             }
         __syncthreads();
+        // each record's place in the chunk's tile order (kBinEmpty: none)
+        unsigned at[PER];
 #pragma unroll
-        for (int k = 0; k < PER; ++k)
+        for (int k = 0; k < PER; ++k) {
+            at[k] = kBinEmpty;
             if (c[k] != kBinEmpty) {
-                const unsigned at = atomicAdd(&off[c[k] >> tile_bits], 1u);
-                stage[at] = c[k];
-                src[at] = (unsigned)(k * kScatterLanes + tid);
+                at[k] = atomicAdd(&off[c[k] >> tile_bits], 1u);
+                stage[at[k]] = c[k];
             }
+        }
         __syncthreads();
         for (unsigned i = (unsigned)tid; i < total; i += kScatterLanes) {
             const unsigned r = stage[i];
             const unsigned t = r >> tile_bits;
-            const unsigned long long g = cur[t] + (i - (t ? off[t - 1] : 0u));
-            sorted[g] = r;
-            const unsigned long long from = c0 + src[i];
-            for (int j = 0; j < nr; ++j)
-                pay_sorted[(unsigned long long)j * stride + g] =
-                    pay[(unsigned long long)j * stride + from];
+            sorted[cur[t] + (i - (t ? off[t - 1] : 0u))] = r;
+        }
+        for (int j = 0; j < nr; ++j) {
+            const PR *src = pay + (unsigned long long)j * stride + c0;
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
+                if (at[k] != kBinEmpty)
+                    pl[at[k]] = src[(unsigned)(k * kScatterLanes + tid)];
+            __syncthreads();
+            PR *dst = pay_sorted + (unsigned long long)j * stride;
+            for (unsigned i = (unsigned)tid; i < total; i += kScatterLanes) {
+                const unsigned t = stage[i] >> tile_bits;
+                dst[cur[t] + (i - (t ? off[t - 1] : 0u))] = pl[i];
+            }
+            __syncthreads();
         }
         __syncthreads();
         for (int t = tid; t < ntiles; t += kScatterLanes)
@@ -538,25 +555,19 @@ int launch_bin_accumulate(const BinScratch &b, unsigned long long n_records,
                                scatter_lds(ntiles, 16), s, b.rec, n_records, per, tile_bits,
                                ntiles, b.hist, b.start, b.sorted);
     } else if (real_bytes == 8) {
-        if (ntiles <= 2048)
-            hipLaunchKernelGGL((bin_scatter_pay_kernel<16, double>), dim3(nblocks),
-                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 16), s, b.rec,
-                               (const double *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
-                               b.start, b.sorted, (double *)b.pay_sorted, stride, r);
-        else
-            hipLaunchKernelGGL((bin_scatter_pay_kernel<8, double>), dim3(nblocks),
-                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 8), s, b.rec,
-                               (const double *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
-                               b.start, b.sorted, (double *)b.pay_sorted, stride, r);
+        hipLaunchKernelGGL((bin_scatter_pay_kernel<8, double>), dim3(nblocks),
+                           dim3(kScatterLanes), scatter_pay_lds(ntiles, 8, 8), s, b.rec,
+                           (const double *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
+                           b.start, b.sorted, (double *)b.pay_sorted, stride, r);
     } else {
-        if (ntiles <= 2048)
+        if (ntiles <= 4096)
             hipLaunchKernelGGL((bin_scatter_pay_kernel<16, float>), dim3(nblocks),
-                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 16), s, b.rec,
+                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 16, 4), s, b.rec,
                                (const float *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
                                b.start, b.sorted, (float *)b.pay_sorted, stride, r);
         else
             hipLaunchKernelGGL((bin_scatter_pay_kernel<8, float>), dim3(nblocks),
-                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 8), s, b.rec,
+                               dim3(kScatterLanes), scatter_pay_lds(ntiles, 8, 4), s, b.rec,
                                (const float *)b.pay, n_records, per, tile_bits, ntiles, b.hist,
                                b.start, b.sorted, (float *)b.pay_sorted, stride, r);
     }
@@ -611,8 +622,9 @@ int prepare_bin_kernels()
     // the scatter kernel: chunks of 32 records a lane up to 4096 tiles, of
     // 16 up to 8192
     static_assert(scatter_lds(4096, 32) <= 160 * 1024, "scatter chunk beyond the LDS");
-    static_assert(scatter_pay_lds(2048, 16) <= 160 * 1024, "scatter chunk beyond the LDS");
-    static_assert(scatter_pay_lds(kBinMaxTiles, 8) <= 160 * 1024, "scatter chunk beyond the LDS");
+    static_assert(scatter_pay_lds(4096, 16, 4) <= 160 * 1024, "scatter chunk beyond the LDS");
+    static_assert(scatter_pay_lds(kBinMaxTiles, 8, 4) <= 160 * 1024, "scatter chunk beyond the LDS");
+    static_assert(scatter_pay_lds(kBinMaxTiles, 8, 8) <= 160 * 1024, "scatter chunk beyond the LDS");
     static_assert(accum_colour_lds() <= 160 * 1024, "colour tile beyond the LDS");
     static_assert(scatter_lds(kBinMaxTiles, 16) <= 160 * 1024, "scatter chunk beyond the LDS");
     if (e == hipSuccess)
@@ -623,20 +635,21 @@ int prepare_bin_kernels()
         e = hipFuncSetAttribute((const void *)bin_scatter_kernel<16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)scatter_lds(kBinMaxTiles, 16));
-    // colour flames: chunks of 16 records a lane up to 2048 tiles, of 8 up
-    // to 8192; the accumulation's tile of counts and four f64 sums
-    const void *pay16[] = {(const void *)bin_scatter_pay_kernel<16, double>,
-                           (const void *)bin_scatter_pay_kernel<16, float>};
-    const void *pay8[] = {(const void *)bin_scatter_pay_kernel<8, double>,
-                          (const void *)bin_scatter_pay_kernel<8, float>};
-    for (const void *k : pay16)
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)scatter_pay_lds(2048, 16));
-    for (const void *k : pay8)
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)scatter_pay_lds(kBinMaxTiles, 8));
+    // colour flames: f32 colours in chunks of 16 records a lane up to 4096
+    // tiles, of 8 up to 8192; f64 colours in chunks of 8; the accumulation's
+    // tile of counts and four f64 sums
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bin_scatter_pay_kernel<16, float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)scatter_pay_lds(4096, 16, 4));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bin_scatter_pay_kernel<8, float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)scatter_pay_lds(kBinMaxTiles, 8, 4));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bin_scatter_pay_kernel<8, double>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)scatter_pay_lds(kBinMaxTiles, 8, 8));
     const void *acc[] = {
         (const void *)bin_accum_colour_kernel<unsigned long long, double, double>,
         (const void *)bin_accum_colour_kernel<unsigned long long, double, float>,

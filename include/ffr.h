@@ -70,8 +70,8 @@ typedef struct ffr_options {
      * counts and colour sums are added from LDS (-1 chosen per render: on
      * for flames without colour whose buffer has at least 2^22 and under
      * 2^28 cells when the lane-per-walker kernel runs, and for a colour
-     * flame once a render of 2^26 samples or more made at least one global
-     * atomic add per sample; 0 off; 1 on where the flame allows it: under
+     * flame once a render of 2^26 samples or more made at least 0.6 global
+     * atomic adds per sample; 0 off; 1 on where the flame allows it: under
      * 2^28 cells without colour, under 2^25 with up to four colours).
      * Counts are the same either way; colour sums differ by the rounding
      * of their order of addition. */

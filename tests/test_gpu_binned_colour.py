@@ -150,8 +150,8 @@ def test_binned_colour_spread_flame_full_size(ffr_mod):
 
 def test_colour_flame_switches_to_binned_after_many_misses(ffr_mod):
     """binned -1: a colour flame renders on the atomic path first; after a
-    render of 2^26 samples or more that made at least one global atomic
-    add per sample (LDS-cache misses, four words each here), its later
+    render of 2^26 samples or more that made at least 0.6 global atomic
+    adds per sample (LDS-cache misses, four words each here), its later
     renders are binned.  flame_1069 (2048^2) misses on about 0.6 of its
     samples (2.45 atomics per sample) and switches; flame_1013, whose
     samples stay in the cache, does not.  Either way the buffer equals one
@@ -165,7 +165,7 @@ def test_colour_flame_switches_to_binned_after_many_misses(ffr_mod):
         b, okb = render(ffr_mod, text, 1 << 26, 0, "fp32u64", 0, settle=53)
         assert oka and okb and a.last_stats.binned == 0
         per = (a.last_stats.atomics - a.last_stats.flush_atomics) / a.last_stats.s_iter
-        assert (per >= 1.0) == switch, per
+        assert (per >= 0.6) == switch, per
         assert a.render(1 << 26) and b.render(1 << 26)
         assert a.last_stats.binned == (1 if switch else 0)
         same(a, b, 2, "fp32u64")
