@@ -95,6 +95,24 @@ HBM_PEAK_GBS = 8000.0
 # The raw values are in profiles/walk_kernel_random_color_3840x2160_hbm.md.
 C2_FP32_TRAFFIC_BYTES = 901.125 * 1024 * 2.0 + 18719063.3125 * 1024 * 1.0
 C2_FP64_TRAFFIC_BYTES = 1001.1875 * 1024 * 2.0 + 18871357.5625 * 1024 * 1.0
+# VALU instructions (SQ_INSTS_VALU, wave64) per headline launch of 2^32
+# samples, by precision and kernel (table-driven or the flame's compiled
+# one), from counter runs of the same launches
+# (profiles/walk_kernel_random_color_3840x2160_valu.md); the compiled FP32
+# kernel was counted only on 2^29-sample runs (x 8 here).  The VALU peak is
+# one wave64 instruction per 4 cycles per SIMD (16 lanes a cycle, FP32 and
+# FP64 alike on this device): 256 CUs x 4 SIMDs x 2.4 GHz / 4.
+C2_VALU_PER_LAUNCH = {("fp32", False): 37062773411, ("fp32", True): 4069185530 * 8,
+                      ("fp64", False): 52.21e9, ("fp64", True): 46.63e9}
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
+
+
+def valu_roofline(precision, res):
+    """the headline launch's VALU issue rate against the device's peak"""
+    n = C2_VALU_PER_LAUNCH[(precision, bool(res["compiled"]))]
+    achieved = n / (res["kernel_ms"] * 1e-3) / 1e9
+    return {"achieved": achieved, "peak": VALU_PEAK_G, "unit": "G VALU instructions/s",
+            "frac": achieved / VALU_PEAK_G, "instructions_per_launch": n}
 
 
 def log(*a):
@@ -520,12 +538,14 @@ def main():
                             warmup=W, bytes_per_plot=32, seed=C2_SEED, settle=SETTLE_FP64,
                             keep=not args.fp32_only)
         fp32["roofline"]["traffic"] = C2_FP32_TRAFFIC_BYTES
+        fp32["valu_roofline"] = valu_roofline("fp32", fp32)
         log("c2 fp32", json.dumps(fp32))
         if not args.fp32_only:
             fp64, h64 = run_gpu(ffr, c2_text, precision="fp64", samples=C2_SAMPLES, steps=S,
                                 warmup=W, bytes_per_plot=32, seed=C2_SEED, keep=True,
                                 image=True, cpu=not args.no_cpu)
             fp64["roofline"]["traffic"] = C2_FP64_TRAFFIC_BYTES
+            fp64["valu_roofline"] = valu_roofline("fp64", fp64)
             l1_c2 = l1(h32, h64)
             del h32
             # the sampling floor: FP64 against FP64 on other streams
@@ -684,13 +704,17 @@ def main():
         },
         "roofline": fp32["roofline"],
         "atomic_roofline": fp32["atomic_roofline"],
+        "valu_roofline": fp32["valu_roofline"],
         "kernel_ms": fp32["kernel_ms"],
+        "compiled": fp32["compiled"],
         "fp64": None if fp64 is None else {
             "value": fp64["value"],
             "ms_per_step": fp64["ms_per_step"],
             "kernel_ms": fp64["kernel_ms"],
+            "compiled": fp64["compiled"],
             "roofline": fp64["roofline"],
             "atomic_roofline": fp64["atomic_roofline"],
+            "valu_roofline": fp64["valu_roofline"],
         },
         "image": None if fp64 is None else fp64.get("image"),
         "l1_fp32_vs_fp64": l1_c2,
