@@ -54,6 +54,21 @@ int fail(int code, const std::string &msg)
 constexpr uint64_t kBinColourMinSamples = 1ull << 26;
 constexpr double kBinColourAtomics = 0.6;
 
+// the kernels take the xform record offsets as constants (XformLay<D>);
+// the table builder must lay records out the same way
+template <int D> bool layout_matches(const XformLayout &l)
+{
+    using K = XformLay<D>;
+    return l.preA == K::preA && l.preb == K::preb && l.postA == K::postA &&
+           l.postb == K::postb && l.speed == K::speed && l.has_color == K::has_color &&
+           l.has_pre == K::has_pre && l.has_post == K::has_post && l.color == K::color;
+}
+bool layout_matches_kernels(const XformLayout &l, int dims)
+{
+    return dims == 1 ? layout_matches<1>(l) : dims == 2 ? layout_matches<2>(l)
+                                             : dims == 3 && layout_matches<3>(l);
+}
+
 struct DevTables {
     void *table = nullptr;
     unsigned long long *masks = nullptr;
@@ -313,6 +328,11 @@ int ffr_create(const char *flame_json, size_t len, const ffr_options *opt_in,
             r->f64 = compile_flame<double>(r->spec);
     } catch (const std::exception &e) {
         return fail(FFR_E_FLAME, e.what());
+    }
+    {
+        const XformLayout &lay = fp32 ? r->f32.lay : r->f64.lay;
+        if (!layout_matches_kernels(lay, r->spec.dims))
+            return fail(FFR_E_UNSUPPORTED, "ffr_create: xform record layout differs from the kernels'");
     }
     HIP_TRY(hipSetDevice(r->device));
     HIP_TRY(hipDeviceGetAttribute(&r->cu_count, hipDeviceAttributeMultiprocessorCount,

@@ -97,6 +97,15 @@ template <typename R> struct WalkArgs : WalkDyn<R> {
         l_has_post, l_stride;
 };
 
+// The xform record layout (flame_model.cpp xform_layout): every offset but
+// the stride depends only on the dimension count, so the kernels take them
+// as constants (immediate offsets of the LDS reads, no address arithmetic)
+template <int D> struct XformLay {
+    static constexpr int preA = 0, preb = D * D, postA = D * D + D, postb = 2 * D * D + D,
+                         speed = 2 * D * D + 2 * D, has_color = speed + 1, has_pre = speed + 2,
+                         has_post = speed + 3, color = speed + 4;
+};
+
 // lanes per workgroup (= walkers per task) of the binned walker kernel
 constexpr int kBinWalkBlock = 256;
 
