@@ -74,7 +74,9 @@ typedef struct ffr_options {
      * atomic adds per sample; 0 off; 1 on where the flame allows it: under
      * 2^28 cells without colour, under 2^25 with up to four colours).
      * Counts are the same either way; colour sums differ by the rounding
-     * of their order of addition. */
+     * of their order of addition.  The first binned render allocates the
+     * pass scratch, kept to ffr_destroy: up to 2 GiB of records, and for
+     * colour flames up to 6 GiB (three FP32 colours) or 12 GiB (FP64). */
     int32_t binned;
     /* the flame compiled into its own kernel (no counterpart in the
      * reference, whose flame is objects with virtual calls): the walker code
