@@ -129,6 +129,21 @@ def test_binned_colour_compiled_walker(ffr_mod, small_passes, mode):
     b.close()
 
 
+@pytest.mark.parametrize("mode", ["fp32u64", "fp32"])
+def test_binned_colour_many_tiles(ffr_mod, small_passes, mode):
+    """a colour buffer of more than 4096 tiles of 4096 cells (4200^2 cells,
+    4307 tiles): the scatter sorts its f32 colours in chunks of 8 records a
+    lane (16 up to 4096 tiles); the buffer equals the atomic path's"""
+    text = TF.color_flame(3, size=4200)
+    print(f"binned colour 4200^2 {mode}", flush=True)
+    a, oka = render(ffr_mod, text, 1 << 21, 1024, mode, 1)
+    b, okb = render(ffr_mod, text, 1 << 21, 1024, mode, 0)
+    assert oka and okb and a.last_stats.binned == 1 and b.last_stats.binned == 0
+    same(a, b, 2, mode)
+    a.close()
+    b.close()
+
+
 def test_binned_colour_spread_flame_full_size(ffr_mod):
     """a batch flame whose samples spread beyond the LDS cache, at its size
     (2048^2, 3 colours, FP32, 8-byte words): the binned buffer equals the
