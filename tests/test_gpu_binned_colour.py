@@ -129,6 +129,38 @@ def test_binned_colour_compiled_walker(ffr_mod, small_passes, mode):
     b.close()
 
 
+def _with_colours(text, r, seed):
+    import json
+    d = json.loads(text)
+    rng = np.random.default_rng(seed)
+    d["color_dimensions"] = r
+    for k, x in enumerate(d["xforms"]):
+        if k != 1:          # xform 1 leaves the colour alone
+            x["color"] = rng.uniform(0, 1, r).tolist()
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32u64"])
+@pytest.mark.parametrize("name", ["tetrahedron_2colours", "one_d_1colour"])
+def test_binned_colour_1d_3d(ffr_mod, oracle_mod, small_passes, name, mode):
+    """colour records of 1-D and 3-D flames: the buffer equals the atomic
+    path's, and the counts the oracle's"""
+    if name == "tetrahedron_2colours":
+        text, dims = _with_colours(TF.tetrahedron(size=48), 2, 11), 3
+    else:
+        text, dims = _with_colours(TF.one_d(size=300, nonlinear=True), 1, 12), 1
+    print(f"binned colour {name} {mode}", flush=True)
+    a, oka = render(ffr_mod, text, 500_000, 1000, mode, 1)
+    b, okb = render(ffr_mod, text, 500_000, 1000, mode, 0)
+    assert oka and okb and a.last_stats.binned == 1 and b.last_stats.binned == 0
+    same(a, b, dims, mode)
+    ref = oracle_mod.OracleFlame(text, MODES[mode], settle=20).render(
+        500_000, 1000, seed=SEED, threads=THREADS)
+    assert np.array_equal(a.counts(a.read_buffer()), ref.counts)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("mode", ["fp32u64", "fp32"])
 def test_binned_colour_many_tiles(ffr_mod, small_passes, mode):
     """a colour buffer of more than 4096 tiles of 4096 cells (4200^2 cells,
