@@ -95,6 +95,10 @@ HBM_PEAK_GBS = 8000.0
 # The raw values are in profiles/walk_kernel_random_color_3840x2160_hbm.md.
 C2_FP32_TRAFFIC_BYTES = 901.125 * 1024 * 2.0 + 18719063.3125 * 1024 * 1.0
 C2_FP64_TRAFFIC_BYTES = 1001.1875 * 1024 * 2.0 + 18871357.5625 * 1024 * 1.0
+# HBM bytes of one binned tetrahedron render (32 passes of 2^28 records):
+# per pass walker 1.10 GB, tile histogram 1.09, column scan 0.03, scatter
+# 2.53, tile accumulation 1.11 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)
+C3_BINNED_TRAFFIC_BYTES = 32 * 5.86e9
 # VALU instructions (SQ_INSTS_VALU, wave64) per headline launch of 2^32
 # samples, by precision and kernel (table-driven or the flame's compiled
 # one), from counter runs of the same launches
@@ -592,9 +596,17 @@ def main():
         if res["binned"]:
             # binned accumulation moves records, not atomics: per sample 4 B
             # written by the walker, the tile histogram's read, the scatter's
-            # read and write, the accumulation's read
+            # read and write, the accumulation's read.  The roofline prices
+            # those bytes (the render's passes together); the measured HBM
+            # traffic of the passes is 5.86 GB per pass of 2^28 records
+            # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 of each kernel,
+            # profiles/bin_kernels_tetrahedron_512.md), 32 passes
             res["record_bytes_per_sample"] = 20
             res["record_traffic_GBs"] = (20 * C3_SAMPLES / (res["kernel_ms"] * 1e-3) / 1e9)
+            res["roofline"] = {"bound": "hbm", "achieved": res["record_traffic_GBs"],
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": res["record_traffic_GBs"] / HBM_PEAK_GBS,
+                               "traffic": C3_BINNED_TRAFFIC_BYTES}
         workloads["tetrahedron_512_fp32"] = res
         log("tetrahedron_512_fp32", json.dumps(res))
 
