@@ -106,8 +106,8 @@ C3_BINNED_TRAFFIC_BYTES = 32 * 5.86e9
 # kernel was counted only on 2^29-sample runs (x 8 here).  The VALU peak is
 # one wave64 instruction per 4 cycles per SIMD (16 lanes a cycle, FP32 and
 # FP64 alike on this device): 256 CUs x 4 SIMDs x 2.4 GHz / 4.
-C2_VALU_PER_LAUNCH = {("fp32", False): 37062773411, ("fp32", True): 4069185530 * 8,
-                      ("fp64", False): 52.21e9, ("fp64", True): 46.63e9}
+C2_VALU_PER_LAUNCH = {("fp32", False): 35077692829, ("fp32", True): 4069185530 * 8,
+                      ("fp64", False): 51868535255, ("fp64", True): 46629032406}
 VALU_PEAK_G = 256 * 4 * 2.4 / 4
 
 
