@@ -218,3 +218,25 @@ def test_colour_flame_switches_to_binned_after_many_misses(ffr_mod):
         same(a, b, 2, "fp32u64")
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("samples,batch", [(1, 0), (37, 5), (1000, 7)])
+def test_binned_colour_ragged_renders(ffr_mod, oracle_mod, samples, batch):
+    """renders of one sample, of fewer walkers than a task's 256 lanes and
+    of a last walker shorter than the others: counts and statistics equal
+    the oracle's, colours the atomic path's; a render of 0 samples leaves
+    the buffer alone"""
+    text = TF.color_flame(3, size=64)
+    a, oka = render(ffr_mod, text, samples, batch, "fp64", 1)
+    b, okb = render(ffr_mod, text, samples, batch, "fp64", 0)
+    assert oka and okb and a.last_stats.binned == 1
+    same(a, b, 2, "fp64")
+    ref = oracle_mod.OracleFlame(text, MODES["fp64"], settle=20).render(
+        samples, batch or a.last_stats.batch_size, seed=SEED, threads=1)
+    assert np.array_equal(a.counts(a.read_buffer()), ref.counts)
+    assert a.samples_iterated == ref.s_iter and a.samples_plotted == ref.s_plot
+    before = a.read_buffer().copy()
+    assert a.render(0)
+    assert np.array_equal(a.read_buffer(), before)
+    a.close()
+    b.close()
