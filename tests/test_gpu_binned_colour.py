@@ -240,3 +240,21 @@ def test_binned_colour_ragged_renders(ffr_mod, oracle_mod, samples, batch):
     assert np.array_equal(a.read_buffer(), before)
     a.close()
     b.close()
+
+
+def test_binned_colour_size_limit(ffr_mod):
+    """colour records address tiles of 4096 cells, at most 8192 of them: a
+    colour buffer of 2^25 cells or more keeps the atomics even when binned
+    accumulation is asked for; one just under (8191 tiles, the scatter's
+    largest LDS) takes it and equals the atomic path"""
+    over = TF.color_flame(3, size=5793)     # 33,558,849 cells > 2^25
+    r, ok = render(ffr_mod, over, 100_000, 1000, "fp32u64", 1)
+    assert ok and r.last_stats.binned == 0
+    r.close()
+    under = TF.color_flame(3, size=5792)    # 33,547,264 cells < 2^25, 8191 tiles
+    a, oka = render(ffr_mod, under, 100_000, 1000, "fp32u64", 1)
+    b, okb = render(ffr_mod, under, 100_000, 1000, "fp32u64", 0)
+    assert oka and okb and a.last_stats.binned == 1 and b.last_stats.binned == 0
+    same(a, b, 2, "fp32u64")
+    a.close()
+    b.close()
