@@ -168,3 +168,20 @@ def test_tetrahedron_512_full_size(ffr_mod):
     same(a, b, 3)
     a.close()
     b.close()
+
+
+def test_binned_over_4096_tiles_and_size_limit(ffr_mod):
+    """count-only buffers of more than 4096 tiles of 2^15 cells (a 12000^2
+    fern, 4395 tiles: the scatter's chunks of 16 records a lane) equal the
+    atomic path; buffers of 2^28 cells and more keep the atomics even when
+    binned accumulation is asked for"""
+    text = TF.fern(size=12000)
+    a, oka = render(ffr_mod, text, 1 << 22, 2048, "fp32", 1)
+    b, okb = render(ffr_mod, text, 1 << 22, 2048, "fp32", 0)
+    assert oka and okb and a.last_stats.binned == 1 and b.last_stats.binned == 0
+    same(a, b, 2)
+    a.close()
+    b.close()
+    r, ok = render(ffr_mod, TF.fern(size=16385), 1 << 20, 1024, "fp32", 1)  # > 2^28 cells
+    assert ok and r.last_stats.binned == 0
+    r.close()
